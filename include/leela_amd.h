@@ -1,0 +1,118 @@
+/*
+ * leela_amd.h — C ABI of the MI355X leaf-evaluation backend.
+ *
+ * This library replaces Leela's OpenCL backend (OpenCL.h / OpenCL.cpp) behind
+ * the unchanged Network API. Each entry point below names the reference
+ * interface it stands in for. Plain pointers and sizes only; no exception or
+ * C++ type crosses this boundary. Every call returns LZ_OK (0) or a negative
+ * status; lz_last_error() gives the message of the last failure on a context.
+ *
+ * Data formats at the boundary
+ *   planes  uint8 [n][32][S*S], values 0/1, plane order as the reference's
+ *           gather_features_policy / gather_features_value (Network.cpp:
+ *           883-1201), index y*S+x, already rotated by the caller exactly as
+ *           get_scored_moves_internal does (Network.cpp:765-773).
+ *   policy  raw = ELU'd output of the last 1-channel conv, float [n][S*S]
+ *           (what the reference hands to softmax, Network.cpp:777,808);
+ *           out = softmax(raw, T) float [n][S*S] (Network.cpp:450-469).
+ *   value   raw = output of the last inner product, float [n];
+ *           out = (1 + tanh(raw)) / 2, float [n] (Network.cpp:704,736).
+ */
+#ifndef LEELA_AMD_H
+#define LEELA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LZ_OK 0
+#define LZ_ERR_ARG (-1)         /* bad argument or shape */
+#define LZ_ERR_HIP (-2)         /* HIP runtime error */
+#define LZ_ERR_STATE (-3)       /* call out of order (e.g. before finalize) */
+#define LZ_ERR_NOMEM (-4)       /* device or host allocation failed */
+#define LZ_ERR_UNSUPPORTED (-5) /* layer table the kernels do not implement */
+
+#define LZ_NET_POLICY 0
+#define LZ_NET_VALUE 1
+
+#define LZ_PREC_FP32 0 /* plain form: FP32 everywhere, FP32 MFMA */
+#define LZ_PREC_BF16 1 /* BF16 trunk on MFMA, FP32 accumulate/epilogue */
+#define LZ_PREC_FP8 2  /* FP8 e4m3 trunk, per-output-channel weight scales */
+
+typedef struct lz_ctx lz_ctx;
+
+/* Opens device `device` (HIP ordinal) and creates an empty context.
+ * Replaces OpenCL::initialize (OpenCL.cpp:764-908), which picks a device
+ * and builds the kernels; here the kernels are precompiled for gfx950. */
+int lz_init(lz_ctx **ctx, int device);
+
+/* Appends a same-padded conv layer (ELU applied to its output) to net
+ * LZ_NET_POLICY or LZ_NET_VALUE. w is OIHW [cout][cin][filter][filter],
+ * b is [cout]; both are copied. Replaces OpenCL_Network::push_convolve
+ * (OpenCL.h:67-77), whose channel count is derived as W/(B*k*k). */
+int lz_push_convolve(lz_ctx *ctx, int net, int filter, int cin, int cout,
+                     const float *w, const float *b);
+
+/* Appends an inner-product layer y = W x + b, ELU iff outputs > 1.
+ * w is [outputs][inputs]. Replaces OpenCL_Network::push_innerproduct
+ * (OpenCL.h:79-87) and the kernel at OpenCL.cpp:407-438. */
+int lz_push_innerproduct(lz_ctx *ctx, int net, int inputs, int outputs,
+                         const float *w, const float *b);
+
+/* Validates both layer tables and uploads the weights in the packed
+ * layouts every precision uses. Must follow the last push and precede any
+ * forward. The reference copies weights at push time (OpenCL.cpp:474-488);
+ * packing for MFMA needs the whole table, hence the extra step. */
+int lz_finalize(lz_ctx *ctx);
+
+/* Softmax temperature of the policy head (cfg_softmax_temp, GTP.cpp:89-93;
+ * default 0.75). */
+int lz_set_softmax_temp(lz_ctx *ctx, float temperature);
+
+/* Batched forward of `net` over n positions of 19x19, host buffers in and
+ * out, synchronous. Replaces OpenCL_Network::forward (OpenCL.cpp:490-569),
+ * which evaluates one position per call and blocks on queue.finish().
+ * raw or out may be NULL when not wanted. */
+int lz_forward(lz_ctx *ctx, int net, int precision, int n,
+               const uint8_t *planes, float *raw, float *out);
+
+/* Same as lz_forward with device-resident buffers, enqueued on `stream`
+ * (a hipStream_t, or NULL for the context's own stream); returns after
+ * enqueueing. Buffers must stay valid until the stream reaches the work. */
+int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
+                      const uint8_t *d_planes, float *d_raw, float *d_out,
+                      void *stream);
+
+/* As lz_forward_device, and also times every kernel launch of the forward
+ * with HIP events recorded on `stream` between launches. Writes up to
+ * max_launches durations (ms, launch order) to launch_ms and the launch
+ * count to *n_launches; synchronises the stream before returning.
+ * Measurement hook; no counterpart in the reference. */
+int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
+                     const uint8_t *d_planes, float *d_raw, float *d_out,
+                     void *stream, float *launch_ms, int max_launches,
+                     int *n_launches);
+
+/* Fills out[0..n) with N(0, stddev^2) draws: Box-Muller over the
+ * reference's xoroshiro128+ (Random.cpp:27-57) seeded with `seed`. The
+ * synthetic He-normal weight sets are made with it because the shipped
+ * weight blobs (NN128.cpp, NNValue.cpp) are not available. */
+int lz_normal_fill(uint32_t seed, size_t n, float stddev, float *out);
+
+/* Device name, as OpenCL::get_device_name (OpenCL.h:121). */
+const char *lz_device_name(lz_ctx *ctx);
+
+/* Message of the last failed call on ctx (or of the last failed lz_init
+ * when ctx is NULL). */
+const char *lz_last_error(lz_ctx *ctx);
+
+/* Frees device memory and the context. */
+void lz_destroy(lz_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

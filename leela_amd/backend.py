@@ -1,0 +1,185 @@
+"""ctypes binding of the C ABI in include/leela_amd.h.
+
+This is the Python mirror of the reference's OpenCL_Network seam
+(OpenCL.h:47-111): push_convolve / push_innerproduct / forward, batched.
+The HIP library must be built (python -m leela_amd.build); there is no
+fallback path — a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libleela_amd.so"
+HEADER = PKG.parent / "include" / "leela_amd.h"
+
+LZ_OK = 0
+LZ_ERR_ARG = -1
+LZ_ERR_HIP = -2
+LZ_ERR_STATE = -3
+LZ_ERR_NOMEM = -4
+LZ_ERR_UNSUPPORTED = -5
+
+NET_POLICY, NET_VALUE = 0, 1
+PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
+PRECISIONS = {"fp32": PREC_FP32, "bf16": PREC_BF16, "fp8": PREC_FP8}
+
+
+class LeelaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library():
+    """Load libleela_amd.so. Raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: the HIP backend is not built "
+            "(run `python -m leela_amd.build`)")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    f32p = C.POINTER(C.c_float)
+    L.lz_init.argtypes = [C.POINTER(vp), C.c_int]
+    L.lz_push_convolve.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   f32p, f32p]
+    L.lz_push_innerproduct.argtypes = [vp, C.c_int, C.c_int, C.c_int, f32p,
+                                       f32p]
+    L.lz_finalize.argtypes = [vp]
+    L.lz_set_softmax_temp.argtypes = [vp, C.c_float]
+    L.lz_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                             C.POINTER(C.c_uint8), f32p, f32p]
+    L.lz_forward_device.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
+                                    vp]
+    L.lz_forward_timed.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
+                                   vp, f32p, C.c_int, C.POINTER(C.c_int)]
+    L.lz_normal_fill.argtypes = [C.c_uint32, C.c_size_t, C.c_float, f32p]
+    L.lz_device_name.argtypes = [vp]
+    L.lz_device_name.restype = C.c_char_p
+    L.lz_last_error.argtypes = [vp]
+    L.lz_last_error.restype = C.c_char_p
+    L.lz_destroy.argtypes = [vp]
+    L.lz_destroy.restype = None
+    _lib = L
+    return L
+
+
+def header_functions() -> list[str]:
+    """Names of the functions include/leela_amd.h declares."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lz_\w+)\s*\(", text)))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, ct=C.c_float):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def normal_fill(seed: int, n: int, stddev: float) -> np.ndarray:
+    out = np.empty(n, np.float32)
+    rc = load_library().lz_normal_fill(seed, n, stddev, _ptr(out))
+    if rc != LZ_OK:
+        raise LeelaError(rc, "lz_normal_fill")
+    return out
+
+
+class Backend:
+    """One lz_ctx: device weights for the policy and value nets."""
+
+    def __init__(self, device: int = 0):
+        self._L = load_library()
+        self._ctx = C.c_void_p()
+        rc = self._L.lz_init(C.byref(self._ctx), device)
+        if rc != LZ_OK:
+            raise LeelaError(rc, self._L.lz_last_error(None).decode())
+
+    def _check(self, rc: int, what: str):
+        if rc != LZ_OK:
+            raise LeelaError(rc, f"{what}: "
+                             f"{self._L.lz_last_error(self._ctx).decode()}")
+
+    @property
+    def device_name(self) -> str:
+        return self._L.lz_device_name(self._ctx).decode()
+
+    def push_convolve(self, net: int, filt: int, cin: int, cout: int, w, b):
+        w, b = _f32(w), _f32(b)
+        self._check(self._L.lz_push_convolve(self._ctx, net, filt, cin, cout,
+                                             _ptr(w), _ptr(b)),
+                    "lz_push_convolve")
+
+    def push_innerproduct(self, net: int, inputs: int, outputs: int, w, b):
+        w, b = _f32(w), _f32(b)
+        self._check(self._L.lz_push_innerproduct(self._ctx, net, inputs,
+                                                 outputs, _ptr(w), _ptr(b)),
+                    "lz_push_innerproduct")
+
+    def load_net(self, net: int, layers):
+        """layers: (kind, filter, cin, cout, w, b) with kind 0=conv 1=FC."""
+        for kind, filt, cin, cout, w, b in layers:
+            if kind == 0:
+                self.push_convolve(net, filt, cin, cout, w, b)
+            else:
+                self.push_innerproduct(net, cin, cout, w, b)
+
+    def finalize(self):
+        self._check(self._L.lz_finalize(self._ctx), "lz_finalize")
+
+    def set_temperature(self, t: float):
+        self._check(self._L.lz_set_softmax_temp(self._ctx, t),
+                    "lz_set_softmax_temp")
+
+    def forward(self, net: int, precision: int, planes: np.ndarray):
+        """planes uint8 [n][32][361] -> (raw, out) float32."""
+        planes = np.ascontiguousarray(planes, dtype=np.uint8)
+        n = planes.shape[0]
+        per = 361 if net == NET_POLICY else 1
+        raw = np.zeros((n, per), np.float32)
+        out = np.zeros((n, per), np.float32)
+        self._check(self._L.lz_forward(self._ctx, net, precision, n,
+                                       _ptr(planes, C.c_uint8), _ptr(raw),
+                                       _ptr(out)), "lz_forward")
+        return raw, out
+
+    def forward_device(self, net: int, precision: int, n: int, d_planes: int,
+                       d_raw: int, d_out: int, stream: int | None = None):
+        """Device-pointer forward (ints are raw device addresses)."""
+        self._check(self._L.lz_forward_device(self._ctx, net, precision, n,
+                                              d_planes, d_raw, d_out, stream),
+                    "lz_forward_device")
+
+    def forward_timed(self, net: int, precision: int, n: int, d_planes: int,
+                      d_raw: int, d_out: int, stream: int | None = None):
+        """Device-pointer forward; returns per-launch durations in ms."""
+        ms = np.zeros(64, np.float32)
+        cnt = C.c_int(0)
+        self._check(self._L.lz_forward_timed(self._ctx, net, precision, n,
+                                             d_planes, d_raw, d_out, stream,
+                                             _ptr(ms), 64, C.byref(cnt)),
+                    "lz_forward_timed")
+        return ms[:cnt.value].copy()
+
+    def close(self):
+        if self._ctx:
+            self._L.lz_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

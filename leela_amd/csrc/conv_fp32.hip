@@ -1,0 +1,306 @@
+// conv_fp32.hip — the FP32 "plain" form of Leela's policy and value nets.
+//
+// One workgroup computes one board x 64 output channels of a same-padded
+// conv as an implicit GEMM on FP32 MFMA (v_mfma_f32_16x16x4_f32):
+//   D[cout x pixels] = W[cout x cin*k*k] * im2col(X)[cin*k*k x pixels]
+// which is the product the reference forms explicitly with im2col + sgemm
+// (Network.cpp:344-393, Im2Col.h:9-50). The input board is staged in LDS
+// one channel chunk at a time with a zero halo, so the im2col matrix is
+// never materialised: a B-operand element is one LDS read at
+// chunk_offset(c, ky, kx) + pixel_offset(y, x). Bias and ELU are fused
+// into the epilogue (Network.cpp:382-392).
+#include "lz_internal.h"
+
+namespace lz {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;           // 4 waves
+constexpr int kNTiles = 24;             // 16-pixel tiles: 384 >= 361
+constexpr int kTilesPerWave = kNTiles / 4;
+constexpr int kMTiles = kF32MBlock / 16;
+
+__device__ __forceinline__ float elu(float v) {
+    return v > 0.0f ? v : (expf(v) - 1.0f);
+}
+
+template <int F>
+struct ConvF32Cfg {
+    static constexpr int CK = (F == 5) ? 8 : 16;  // input channels per chunk
+    static constexpr int KC = CK * F * F;         // K rows per chunk
+    static constexpr int SP = kMaxSide + 2 * (F / 2);
+};
+
+template <int F, typename InT>
+__global__ __launch_bounds__(kThreads) void conv_f32_kernel(
+    const InT *__restrict__ in, const float *__restrict__ wpk,
+    const float *__restrict__ bias, float *__restrict__ out, int cin,
+    int cout, int S) {
+    using C = ConvF32Cfg<F>;
+    constexpr int PAD = F / 2;
+    __shared__ float s_in[C::CK * C::SP * C::SP];
+    __shared__ f32x4 s_w4[C::KC * kF32MBlock / 4];
+    __shared__ int s_koff[C::KC];
+    const float *s_w = reinterpret_cast<const float *>(s_w4);
+
+    const int board = blockIdx.x;
+    const int mb = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int P = S * S;
+    const int SPs = S + 2 * PAD;  // padded side of this board
+    const int plane = SPs * SPs;
+
+    // Offsets of the (c, ky, kx) rows of one chunk inside s_in.
+    for (int i = tid; i < C::KC; i += kThreads) {
+        const int c = i / (F * F), r = i % (F * F);
+        s_koff[i] = c * plane + (r / F) * SPs + (r % F);
+    }
+    // Offset of this lane's pixel in each of its wave's N tiles. Columns
+    // past the board read in-bounds garbage and are never stored.
+    int poff[kTilesPerWave];
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; t++) {
+        const int j = (wave * kTilesPerWave + t) * 16 + (lane & 15);
+        poff[t] = j < P ? (j / S) * SPs + (j % S) : 0;
+    }
+
+    f32x4 acc[kMTiles][kTilesPerWave];
+#pragma unroll
+    for (int m = 0; m < kMTiles; m++)
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; t++) acc[m][t] = f32x4{0, 0, 0, 0};
+
+    const InT *src = in + (size_t)board * cin * P;
+    const int K = cin * F * F;
+    const f32x4 *wsrc4 = reinterpret_cast<const f32x4 *>(
+        wpk + (size_t)mb * K * kF32MBlock);
+    const int kq = lane >> 4;
+    const int ml = lane & 15;
+
+    for (int c0 = 0; c0 < cin; c0 += C::CK) {
+        __syncthreads();
+        for (int i = tid; i < C::CK * plane; i += kThreads) {
+            const int c = i / plane, r = i % plane;
+            const int y = r / SPs - PAD, x = r % SPs - PAD;
+            float v = 0.0f;
+            if (y >= 0 && y < S && x >= 0 && x < S)
+                v = (float)src[(size_t)(c0 + c) * P + y * S + x];
+            s_in[i] = v;
+        }
+        const f32x4 *wchunk = wsrc4 + (size_t)c0 * F * F * kF32MBlock / 4;
+        for (int i = tid; i < C::KC * kF32MBlock / 4; i += kThreads)
+            s_w4[i] = wchunk[i];
+        __syncthreads();
+
+        for (int ks = 0; ks < C::KC; ks += 4) {
+            const int k = ks + kq;
+            const int ko = s_koff[k];
+            float a[kMTiles], b[kTilesPerWave];
+#pragma unroll
+            for (int m = 0; m < kMTiles; m++)
+                a[m] = s_w[k * kF32MBlock + m * 16 + ml];
+#pragma unroll
+            for (int t = 0; t < kTilesPerWave; t++) b[t] = s_in[ko + poff[t]];
+#pragma unroll
+            for (int m = 0; m < kMTiles; m++)
+#pragma unroll
+                for (int t = 0; t < kTilesPerWave; t++)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        a[m], b[t], acc[m][t], 0, 0, 0);
+        }
+    }
+
+    // D layout of 16x16x4: lane holds rows 4*(lane/16)+r of column lane%16.
+    float *dst = out + (size_t)board * cout * P;
+#pragma unroll
+    for (int m = 0; m < kMTiles; m++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int o = mb * kF32MBlock + m * 16 + 4 * (lane >> 4) + r;
+            if (o >= cout) continue;
+            const float bo = bias[o];
+#pragma unroll
+            for (int t = 0; t < kTilesPerWave; t++) {
+                const int j = (wave * kTilesPerWave + t) * 16 + (lane & 15);
+                if (j < P) dst[(size_t)o * P + j] = elu(acc[m][t][r] + bo);
+            }
+        }
+    }
+}
+
+// ---- 1-channel head convs -------------------------------------------------
+
+constexpr int kHeadCK = 16;
+constexpr int kHeadSP = kMaxSide + 2;
+
+// Computes the ELU'd 3x3 (cin -> 1) conv of one board into s_out[0..P).
+// Each thread owns pixels tid and tid + 256.
+__device__ void head_conv3(const float *__restrict__ src,
+                           const float *__restrict__ w, float b, int cin,
+                           int S, float *s_in, float *s_w, float *s_out) {
+    const int tid = threadIdx.x;
+    const int P = S * S, SPs = S + 2, plane = SPs * SPs;
+    float acc0 = 0.0f, acc1 = 0.0f;
+    const int p0 = tid, p1 = tid + kThreads;
+    const int o0 = p0 < P ? (p0 / S) * SPs + p0 % S : 0;
+    const int o1 = p1 < P ? (p1 / S) * SPs + p1 % S : 0;
+    for (int c0 = 0; c0 < cin; c0 += kHeadCK) {
+        __syncthreads();
+        for (int i = tid; i < kHeadCK * plane; i += kThreads) {
+            const int c = i / plane, r = i % plane;
+            const int y = r / SPs - 1, x = r % SPs - 1;
+            float v = 0.0f;
+            if (y >= 0 && y < S && x >= 0 && x < S)
+                v = src[(size_t)(c0 + c) * P + y * S + x];
+            s_in[i] = v;
+        }
+        for (int i = tid; i < kHeadCK * 9; i += kThreads)
+            s_w[i] = w[c0 * 9 + i];
+        __syncthreads();
+        for (int c = 0; c < kHeadCK; c++) {
+            const float *pl = s_in + c * plane;
+#pragma unroll
+            for (int ky = 0; ky < 3; ky++)
+#pragma unroll
+                for (int kx = 0; kx < 3; kx++) {
+                    const float wv = s_w[c * 9 + ky * 3 + kx];
+                    acc0 += wv * pl[o0 + ky * SPs + kx];
+                    acc1 += wv * pl[o1 + ky * SPs + kx];
+                }
+        }
+    }
+    if (p0 < P) s_out[p0] = elu(acc0 + b);
+    if (p1 < P) s_out[p1] = elu(acc1 + b);
+    __syncthreads();
+}
+
+__device__ float block_reduce(float v, bool is_max, float *s_red) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(v, off, 64);
+        v = is_max ? fmaxf(v, o) : v + o;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) s_red[wave] = v;
+    __syncthreads();
+    float r = s_red[0];
+    for (int i = 1; i < kThreads / 64; i++)
+        r = is_max ? fmaxf(r, s_red[i]) : r + s_red[i];
+    return r;
+}
+
+__global__ __launch_bounds__(kThreads) void policy_head_f32_kernel(
+    const float *__restrict__ in, const float *__restrict__ w,
+    const float *__restrict__ b, float *__restrict__ raw,
+    float *__restrict__ out, int cin, int S, float temperature) {
+    __shared__ float s_in[kHeadCK * kHeadSP * kHeadSP];
+    __shared__ float s_w[kHeadCK * 9];
+    __shared__ float s_logit[kMaxSide * kMaxSide];
+    __shared__ float s_red[kThreads / 64];
+    const int board = blockIdx.x, tid = threadIdx.x, P = S * S;
+    head_conv3(in + (size_t)board * cin * P, w, b[0], cin, S, s_in, s_w,
+               s_logit);
+    // Network::softmax (Network.cpp:450-469): exp(x/T - max/T) / sum.
+    float mx = -INFINITY;
+    for (int p = tid; p < P; p += kThreads) mx = fmaxf(mx, s_logit[p]);
+    mx = block_reduce(mx, true, s_red);
+    const float alpha = mx / temperature;
+    float e[2] = {0.0f, 0.0f};
+    float sum = 0.0f;
+    for (int i = 0, p = tid; p < P; p += kThreads, i++) {
+        e[i] = expf(s_logit[p] / temperature - alpha);
+        sum += e[i];
+    }
+    sum = block_reduce(sum, false, s_red);
+    for (int i = 0, p = tid; p < P; p += kThreads, i++) {
+        if (raw) raw[(size_t)board * P + p] = s_logit[p];
+        if (out) out[(size_t)board * P + p] = e[i] / sum;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void value_head_f32_kernel(
+    const float *__restrict__ in, const float *__restrict__ w,
+    const float *__restrict__ b, const float *__restrict__ fc1_wt,
+    const float *__restrict__ fc1_b, const float *__restrict__ fc2_w,
+    const float *__restrict__ fc2_b, float *__restrict__ raw,
+    float *__restrict__ out, int cin, int S, int hidden) {
+    __shared__ float s_in[kHeadCK * kHeadSP * kHeadSP];
+    __shared__ float s_w[kHeadCK * 9];
+    __shared__ float s_h0[kMaxSide * kMaxSide];
+    __shared__ float s_red[kThreads / 64];
+    const int board = blockIdx.x, tid = threadIdx.x, P = S * S;
+    head_conv3(in + (size_t)board * cin * P, w, b[0], cin, S, s_in, s_w,
+               s_h0);
+    // innerproduct<361, 256> with ELU, then innerproduct<256, 1> without
+    // (Network.cpp:395-423, 733-734).
+    float part = 0.0f;
+    for (int o = tid; o < hidden; o += kThreads) {
+        float acc = 0.0f;
+        for (int i = 0; i < P; i++) acc += fc1_wt[(size_t)i * hidden + o] * s_h0[i];
+        part += fc2_w[o] * elu(acc + fc1_b[o]);
+    }
+    const float v = block_reduce(part, false, s_red) + fc2_b[0];
+    if (tid == 0) {
+        if (raw) raw[board] = v;
+        if (out) out[board] = (1.0f + tanhf(v)) / 2.0f;
+    }
+}
+
+template <int F, typename InT>
+hipError_t launch_conv(const void *in, const float *wpk, const float *bias,
+                       float *out, int n, int cin, int cout, int S,
+                       hipStream_t stream) {
+    if (cin % ConvF32Cfg<F>::CK != 0) return hipErrorInvalidValue;
+    dim3 grid(n, (cout + kF32MBlock - 1) / kF32MBlock);
+    hipLaunchKernelGGL((conv_f32_kernel<F, InT>), grid, dim3(kThreads), 0,
+                       stream, static_cast<const InT *>(in), wpk, bias, out,
+                       cin, cout, S);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t conv_f32(int filter, bool u8_input, const void *in,
+                    const float *wpk, const float *bias, float *out, int n,
+                    int cin, int cout, int S, hipStream_t stream) {
+    if (S < 1 || S > kMaxSide || n <= 0) return hipErrorInvalidValue;
+    if (filter == 3)
+        return u8_input ? launch_conv<3, uint8_t>(in, wpk, bias, out, n, cin,
+                                                  cout, S, stream)
+                        : launch_conv<3, float>(in, wpk, bias, out, n, cin,
+                                                cout, S, stream);
+    if (filter == 5)
+        return u8_input ? launch_conv<5, uint8_t>(in, wpk, bias, out, n, cin,
+                                                  cout, S, stream)
+                        : launch_conv<5, float>(in, wpk, bias, out, n, cin,
+                                                cout, S, stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t policy_head_f32(const float *in, const float *w, const float *b,
+                           float *raw, float *out, int n, int cin, int S,
+                           float temperature, hipStream_t stream) {
+    if (S < 1 || S > kMaxSide || n <= 0 || cin % kHeadCK != 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(policy_head_f32_kernel, dim3(n), dim3(kThreads), 0,
+                       stream, in, w, b, raw, out, cin, S, temperature);
+    return hipGetLastError();
+}
+
+hipError_t value_head_f32(const float *in, const float *w, const float *b,
+                          const float *fc1_wt, const float *fc1_b,
+                          const float *fc2_w, const float *fc2_b, float *raw,
+                          float *out, int n, int cin, int S, int hidden,
+                          hipStream_t stream) {
+    if (S < 1 || S > kMaxSide || n <= 0 || cin % kHeadCK != 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(value_head_f32_kernel, dim3(n), dim3(kThreads), 0,
+                       stream, in, w, b, fc1_wt, fc1_b, fc2_w, fc2_b, raw,
+                       out, cin, S, hidden);
+    return hipGetLastError();
+}
+
+}  // namespace lz

@@ -1,0 +1,302 @@
+// lz_abi.cpp — the C ABI (include/leela_amd.h): context, layer tables,
+// weight packing and per-precision forward dispatch.
+//
+// Reference seam being replaced: OpenCL_Network / OpenCL (OpenCL.h:47-123,
+// OpenCL.cpp:474-744). The reference keeps one in-order queue and one set
+// of buffers per search thread and evaluates one position per forward; this
+// context owns one stream and batch-sized workspaces, and a forward covers
+// n positions in one pass over the layer table.
+#include "leela_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lz_internal.h"
+#include "lz_net.h"
+
+namespace {
+
+std::mutex g_init_err_mu;
+std::string g_init_err;
+
+void set_init_err(const std::string &s) {
+    std::lock_guard<std::mutex> g(g_init_err_mu);
+    g_init_err = s;
+}
+
+}  // namespace
+
+struct lz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    std::string err;
+    std::mutex mu;
+    lz::Net net[2];
+    lz::Workspace ws;
+    float temperature = 0.75f;
+    bool finalized = false;
+
+    int fail(int code, const char *fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+extern "C" {
+
+int lz_init(lz_ctx **out, int device) {
+    if (!out) return LZ_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        set_init_err(std::string("no HIP device: ") + hipGetErrorString(e));
+        return LZ_ERR_HIP;
+    }
+    if (device < 0 || device >= count) {
+        set_init_err("device ordinal out of range");
+        return LZ_ERR_ARG;
+    }
+    if ((e = hipSetDevice(device)) != hipSuccess) {
+        set_init_err(std::string("hipSetDevice: ") + hipGetErrorString(e));
+        return LZ_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) {
+        set_init_err(std::string("hipGetDeviceProperties: ") +
+                     hipGetErrorString(e));
+        return LZ_ERR_HIP;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_init_err(std::string("kernels are built for gfx950, device is ") +
+                     prop.gcnArchName);
+        return LZ_ERR_UNSUPPORTED;
+    }
+    lz_ctx *ctx = new (std::nothrow) lz_ctx();
+    if (!ctx) return LZ_ERR_NOMEM;
+    ctx->device = device;
+    ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+                std::to_string(prop.multiProcessorCount) + " CUs)";
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) !=
+        hipSuccess) {
+        set_init_err(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        delete ctx;
+        return LZ_ERR_HIP;
+    }
+    *out = ctx;
+    return LZ_OK;
+}
+
+int lz_push_convolve(lz_ctx *ctx, int net, int filter, int cin, int cout,
+                     const float *w, const float *b) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "push after lz_finalize");
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return ctx->fail(LZ_ERR_ARG, "net must be LZ_NET_POLICY or LZ_NET_VALUE");
+    if (!w || !b || cin <= 0 || cout <= 0 || (filter != 3 && filter != 5))
+        return ctx->fail(LZ_ERR_ARG, "bad conv layer (filter %d, %d->%d)",
+                         filter, cin, cout);
+    lz::Layer L;
+    L.kind = lz::Layer::kConv;
+    L.filter = filter;
+    L.cin = cin;
+    L.cout = cout;
+    L.w.assign(w, w + (size_t)cout * cin * filter * filter);
+    L.b.assign(b, b + cout);
+    ctx->net[net].layers.push_back(std::move(L));
+    return LZ_OK;
+}
+
+int lz_push_innerproduct(lz_ctx *ctx, int net, int inputs, int outputs,
+                         const float *w, const float *b) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "push after lz_finalize");
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return ctx->fail(LZ_ERR_ARG, "net must be LZ_NET_POLICY or LZ_NET_VALUE");
+    if (!w || !b || inputs <= 0 || outputs <= 0)
+        return ctx->fail(LZ_ERR_ARG, "bad inner product %d->%d", inputs,
+                         outputs);
+    lz::Layer L;
+    L.kind = lz::Layer::kFC;
+    L.filter = 0;
+    L.cin = inputs;
+    L.cout = outputs;
+    L.w.assign(w, w + (size_t)outputs * inputs);
+    L.b.assign(b, b + outputs);
+    ctx->net[net].layers.push_back(std::move(L));
+    return LZ_OK;
+}
+
+int lz_finalize(lz_ctx *ctx) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->finalized) return ctx->fail(LZ_ERR_STATE, "already finalized");
+    hipSetDevice(ctx->device);
+    for (int k = 0; k < 2; k++) {
+        std::string msg;
+        int rc = lz::validate_net(ctx->net[k], k, &msg);
+        if (rc != LZ_OK) return ctx->fail(rc, "%s net: %s",
+                                          k == 0 ? "policy" : "value",
+                                          msg.c_str());
+    }
+    for (int k = 0; k < 2; k++) {
+        std::string msg;
+        int rc = lz::upload_net(ctx->net[k], k, &msg);
+        if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    }
+    ctx->finalized = true;
+    return LZ_OK;
+}
+
+int lz_set_softmax_temp(lz_ctx *ctx, float temperature) {
+    if (!ctx) return LZ_ERR_ARG;
+    if (!(temperature > 0.0f) || !std::isfinite(temperature))
+        return ctx->fail(LZ_ERR_ARG, "temperature must be positive");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->temperature = temperature;
+    return LZ_OK;
+}
+
+static int forward_locked(lz_ctx *ctx, int net, int precision, int n,
+                          const uint8_t *d_planes, float *d_raw, float *d_out,
+                          hipStream_t stream,
+                          lz::LaunchTimer *timer = nullptr) {
+    if (!ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return ctx->fail(LZ_ERR_ARG, "bad net id %d", net);
+    if (precision != LZ_PREC_FP32 && precision != LZ_PREC_BF16 &&
+        precision != LZ_PREC_FP8)
+        return ctx->fail(LZ_ERR_ARG, "bad precision %d", precision);
+    if (n < 0) return ctx->fail(LZ_ERR_ARG, "negative batch");
+    if (n == 0) return LZ_OK;
+    if (!d_planes) return ctx->fail(LZ_ERR_ARG, "planes is NULL");
+    std::string msg;
+    int rc = lz::ensure_workspace(ctx->ws, ctx->net, n, &msg);
+    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    rc = lz::run_net(ctx->net[net], net, precision, n, d_planes, d_raw, d_out,
+                     ctx->temperature, ctx->ws, stream, &msg, timer);
+    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    return LZ_OK;
+}
+
+int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
+                     const uint8_t *d_planes, float *d_raw, float *d_out,
+                     void *stream, float *launch_ms, int max_launches,
+                     int *n_launches) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipSetDevice(ctx->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    lz::LaunchTimer timer;
+    int rc = forward_locked(ctx, net, precision, n, d_planes, d_raw, d_out, s,
+                            &timer);
+    if (rc != LZ_OK) return rc;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return ctx->fail(LZ_ERR_HIP, "timed forward: %s", hipGetErrorString(e));
+    const int count = timer.events.empty() ? 0 : (int)timer.events.size() - 1;
+    if (n_launches) *n_launches = count;
+    for (int i = 0; i < count && i < max_launches && launch_ms; i++) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, timer.events[i], timer.events[i + 1]);
+        if (e != hipSuccess)
+            return ctx->fail(LZ_ERR_HIP, "event time: %s", hipGetErrorString(e));
+        launch_ms[i] = ms;
+    }
+    return LZ_OK;
+}
+
+int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
+                      const uint8_t *d_planes, float *d_raw, float *d_out,
+                      void *stream) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipSetDevice(ctx->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    return forward_locked(ctx, net, precision, n, d_planes, d_raw, d_out, s);
+}
+
+int lz_forward(lz_ctx *ctx, int net, int precision, int n,
+               const uint8_t *planes, float *raw, float *out) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (n < 0) return ctx->fail(LZ_ERR_ARG, "negative batch");
+    if (n == 0) return LZ_OK;
+    if (!planes) return ctx->fail(LZ_ERR_ARG, "planes is NULL");
+    if (!ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
+    hipSetDevice(ctx->device);
+    std::string msg;
+    int rc = lz::ensure_workspace(ctx->ws, ctx->net, n, &msg);
+    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    lz::Workspace &ws = ctx->ws;
+    const lz::Net &N = ctx->net[net < 0 || net > 1 ? 0 : net];
+    const int cin0 = N.layers.empty() ? 0 : N.layers[0].cin;
+    const size_t in_bytes = (size_t)n * cin0 * N.side * N.side;
+    hipError_t e = hipMemcpyAsync(ws.planes, planes, in_bytes,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess)
+        return ctx->fail(LZ_ERR_HIP, "H2D planes: %s", hipGetErrorString(e));
+    rc = forward_locked(ctx, net, precision, n, ws.planes, ws.raw, ws.out,
+                        ctx->stream);
+    if (rc != LZ_OK) return rc;
+    const size_t per = net == LZ_NET_POLICY ? (size_t)N.side * N.side : 1;
+    if (raw &&
+        (e = hipMemcpyAsync(raw, ws.raw, sizeof(float) * per * n,
+                            hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
+        return ctx->fail(LZ_ERR_HIP, "D2H raw: %s", hipGetErrorString(e));
+    if (out &&
+        (e = hipMemcpyAsync(out, ws.out, sizeof(float) * per * n,
+                            hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
+        return ctx->fail(LZ_ERR_HIP, "D2H out: %s", hipGetErrorString(e));
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return ctx->fail(LZ_ERR_HIP, "forward: %s", hipGetErrorString(e));
+    return LZ_OK;
+}
+
+int lz_normal_fill(uint32_t seed, size_t n, float stddev, float *out) {
+    if (!out && n) return LZ_ERR_ARG;
+    lz::normal_fill(seed, n, stddev, out);
+    return LZ_OK;
+}
+
+const char *lz_device_name(lz_ctx *ctx) {
+    return ctx ? ctx->name.c_str() : "";
+}
+
+const char *lz_last_error(lz_ctx *ctx) {
+    if (ctx) return ctx->err.c_str();
+    std::lock_guard<std::mutex> g(g_init_err_mu);
+    return g_init_err.c_str();
+}
+
+void lz_destroy(lz_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    lz::free_net(ctx->net[0]);
+    lz::free_net(ctx->net[1]);
+    lz::free_workspace(ctx->ws);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // extern "C"

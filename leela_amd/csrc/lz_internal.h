@@ -1,0 +1,36 @@
+// lz_internal.h — shared declarations of the HIP backend (not part of the
+// C ABI). Kernels are compiled for gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lz {
+
+// Largest board side the kernels accept (the reference nets are 19x19 only,
+// Network.cpp:541-543,591-594,627-629).
+constexpr int kMaxSide = 19;
+
+// ---- FP32 plain form (conv_fp32.hip) --------------------------------------
+// Output-channel block of one FP32 conv workgroup; packed weights are laid
+// out [cout/64][cin*k*k][64] with zero rows past cout.
+constexpr int kF32MBlock = 64;
+
+hipError_t conv_f32(int filter, bool u8_input, const void *in,
+                    const float *wpk, const float *bias, float *out, int n,
+                    int cin, int cout, int S, hipStream_t stream);
+
+// Last 1-channel conv + ELU + softmax(T): raw/out [n][S*S].
+hipError_t policy_head_f32(const float *in, const float *w, const float *b,
+                           float *raw, float *out, int n, int cin, int S,
+                           float temperature, hipStream_t stream);
+
+// Last 1-channel conv + ELU, FC (S*S -> hidden) + ELU, FC (hidden -> 1),
+// (1 + tanh) / 2. fc1_wt is the first FC transposed: [S*S][hidden].
+hipError_t value_head_f32(const float *in, const float *w, const float *b,
+                          const float *fc1_wt, const float *fc1_b,
+                          const float *fc2_w, const float *fc2_b, float *raw,
+                          float *out, int n, int cin, int S, int hidden,
+                          hipStream_t stream);
+
+}  // namespace lz

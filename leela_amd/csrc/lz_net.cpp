@@ -1,0 +1,242 @@
+// lz_net.cpp — validation, weight packing/upload and the per-precision
+// layer schedule of the policy and value nets.
+#include "lz_net.h"
+
+#include <cmath>
+
+#include "leela_amd.h"
+#include "lz_internal.h"
+#include "lz_random.h"
+
+namespace lz {
+namespace {
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+    return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (count ? count : 1));
+}
+
+template <typename T>
+hipError_t upload(T **p, const std::vector<T> &h) {
+    hipError_t e = dalloc(p, h.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+}
+
+void dfree(void *p) {
+    if (p) hipFree(p);
+}
+
+}  // namespace
+
+int validate_net(Net &net, int which, std::string *msg) {
+    auto &L = net.layers;
+    int i = 0;
+    while (i < (int)L.size() && L[i].kind == Layer::kConv) i++;
+    const int nconv = i;
+    if (nconv < 2) {
+        *msg = "needs at least one trunk conv and a 1-channel head conv";
+        return LZ_ERR_UNSUPPORTED;
+    }
+    net.n_trunk = nconv - 1;
+    net.head = nconv - 1;
+    net.side = 19;
+    net.max_ch = L[0].cin;
+    for (int k = 0; k < nconv; k++) {
+        if (k > 0 && L[k].cin != L[k - 1].cout) {
+            *msg = "conv " + std::to_string(k) + " input channels do not "
+                   "match the previous layer's outputs";
+            return LZ_ERR_ARG;
+        }
+        const int ck = L[k].filter == 5 ? 8 : 16;
+        if (L[k].cin % ck != 0) {
+            *msg = "conv " + std::to_string(k) + ": input channels must be a "
+                   "multiple of " + std::to_string(ck);
+            return LZ_ERR_UNSUPPORTED;
+        }
+        if (k < nconv - 1 && L[k].cout > 128) {
+            *msg = "trunk convs wider than 128 channels are not built";
+            return LZ_ERR_UNSUPPORTED;
+        }
+        if (L[k].cout > net.max_ch) net.max_ch = L[k].cout;
+    }
+    if (L[net.head].cout != 1 || L[net.head].filter != 3) {
+        *msg = "the last conv must be 3x3 with one output channel";
+        return LZ_ERR_UNSUPPORTED;
+    }
+    const int P = net.side * net.side;
+    if (which == LZ_NET_POLICY) {
+        if ((int)L.size() != nconv) {
+            *msg = "policy net takes no inner-product layers";
+            return LZ_ERR_UNSUPPORTED;
+        }
+    } else {
+        if ((int)L.size() != nconv + 2 || L[nconv].kind != Layer::kFC ||
+            L[nconv + 1].kind != Layer::kFC) {
+            *msg = "value net must end in exactly two inner products";
+            return LZ_ERR_UNSUPPORTED;
+        }
+        net.fc1 = nconv;
+        net.fc2 = nconv + 1;
+        if (L[net.fc1].cin != P || L[net.fc2].cin != L[net.fc1].cout ||
+            L[net.fc2].cout != 1) {
+            *msg = "value inner products must be (S*S -> H) and (H -> 1)";
+            return LZ_ERR_ARG;
+        }
+    }
+    return LZ_OK;
+}
+
+int upload_net(Net &net, int which, std::string *msg) {
+    (void)which;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < (int)net.layers.size() && e == hipSuccess; k++) {
+        Layer &L = net.layers[k];
+        std::vector<float> packed;
+        if (L.kind == Layer::kConv && k < net.head) {
+            // [cout/64][K][64]: a workgroup's 64 output channels for one K
+            // row are contiguous, so the LDS copy is a straight stream.
+            const int K = L.cin * L.filter * L.filter;
+            const int mbs = (L.cout + kF32MBlock - 1) / kF32MBlock;
+            packed.assign((size_t)mbs * K * kF32MBlock, 0.0f);
+            for (int o = 0; o < L.cout; o++)
+                for (int kk = 0; kk < K; kk++)
+                    packed[((size_t)(o / kF32MBlock) * K + kk) * kF32MBlock +
+                           o % kF32MBlock] = L.w[(size_t)o * K + kk];
+        } else if (L.kind == Layer::kFC && k == net.fc1) {
+            packed.assign(L.w.size(), 0.0f);
+            for (int o = 0; o < L.cout; o++)
+                for (int i = 0; i < L.cin; i++)
+                    packed[(size_t)i * L.cout + o] = L.w[(size_t)o * L.cin + i];
+        } else {
+            packed = L.w;
+        }
+        e = upload(&L.d_w32, packed);
+        if (e == hipSuccess) e = upload(&L.d_b, L.b);
+    }
+    if (e != hipSuccess) {
+        *msg = std::string("weight upload: ") + hipGetErrorString(e);
+        return LZ_ERR_HIP;
+    }
+    return LZ_OK;
+}
+
+void free_net(Net &net) {
+    for (auto &L : net.layers) {
+        dfree(L.d_w32);
+        dfree(L.d_b);
+        L.d_w32 = L.d_b = nullptr;
+    }
+}
+
+int ensure_workspace(Workspace &ws, const Net nets[2], int n,
+                     std::string *msg) {
+    if (n <= ws.cap) return LZ_OK;
+    free_workspace(ws);
+    int ch = 32;
+    int cin0 = 0;
+    for (int k = 0; k < 2; k++) {
+        if (nets[k].max_ch > ch) ch = nets[k].max_ch;
+        if (!nets[k].layers.empty() && nets[k].layers[0].cin > cin0)
+            cin0 = nets[k].layers[0].cin;
+    }
+    const size_t P = (size_t)kMaxSide * kMaxSide;
+    hipError_t e = dalloc(&ws.planes, (size_t)n * cin0 * P);
+    if (e == hipSuccess) e = dalloc(&ws.act[0], (size_t)n * ch * P);
+    if (e == hipSuccess) e = dalloc(&ws.act[1], (size_t)n * ch * P);
+    if (e == hipSuccess) e = dalloc(&ws.raw, (size_t)n * P);
+    if (e == hipSuccess) e = dalloc(&ws.out, (size_t)n * P);
+    if (e != hipSuccess) {
+        free_workspace(ws);
+        *msg = std::string("workspace allocation: ") + hipGetErrorString(e);
+        return LZ_ERR_NOMEM;
+    }
+    ws.cap = n;
+    return LZ_OK;
+}
+
+void free_workspace(Workspace &ws) {
+    dfree(ws.planes);
+    dfree(ws.act[0]);
+    dfree(ws.act[1]);
+    dfree(ws.raw);
+    dfree(ws.out);
+    ws = Workspace();
+}
+
+hipError_t LaunchTimer::mark(hipStream_t s) {
+    hipEvent_t ev;
+    hipError_t e = hipEventCreate(&ev);
+    if (e != hipSuccess) return e;
+    events.push_back(ev);
+    return hipEventRecord(ev, s);
+}
+
+LaunchTimer::~LaunchTimer() {
+    for (auto ev : events) hipEventDestroy(ev);
+}
+
+static int run_fp32(const Net &net, int which, int n, const uint8_t *planes,
+                    float *raw, float *out, float temperature, Workspace &ws,
+                    hipStream_t s, std::string *msg, LaunchTimer *timer) {
+    const int S = net.side;
+    const void *x = planes;
+    bool u8 = true;
+    hipError_t e = timer ? timer->mark(s) : hipSuccess;
+    for (int k = 0; k < net.n_trunk && e == hipSuccess; k++) {
+        const Layer &L = net.layers[k];
+        float *y = ws.act[k & 1];
+        e = conv_f32(L.filter, u8, x, L.d_w32, L.d_b, y, n, L.cin, L.cout, S,
+                     s);
+        if (e == hipSuccess && timer) e = timer->mark(s);
+        x = y;
+        u8 = false;
+    }
+    const Layer &H = net.layers[net.head];
+    if (e == hipSuccess) {
+        if (which == LZ_NET_POLICY) {
+            e = policy_head_f32(static_cast<const float *>(x), H.d_w32, H.d_b,
+                                raw, out, n, H.cin, S, temperature, s);
+        } else {
+            const Layer &F1 = net.layers[net.fc1];
+            const Layer &F2 = net.layers[net.fc2];
+            e = value_head_f32(static_cast<const float *>(x), H.d_w32, H.d_b,
+                               F1.d_w32, F1.d_b, F2.d_w32, F2.d_b, raw, out, n,
+                               H.cin, S, F1.cout, s);
+        }
+        if (e == hipSuccess && timer) e = timer->mark(s);
+    }
+    if (e != hipSuccess) {
+        *msg = std::string("FP32 forward launch: ") + hipGetErrorString(e);
+        return LZ_ERR_HIP;
+    }
+    return LZ_OK;
+}
+
+int run_net(const Net &net, int which, int precision, int n,
+            const uint8_t *d_planes, float *d_raw, float *d_out,
+            float temperature, Workspace &ws, hipStream_t stream,
+            std::string *msg, LaunchTimer *timer) {
+    if (precision == LZ_PREC_FP32)
+        return run_fp32(net, which, n, d_planes, d_raw, d_out, temperature,
+                        ws, stream, msg, timer);
+    *msg = "precision not built yet";
+    return LZ_ERR_UNSUPPORTED;
+}
+
+// Box-Muller over the reference RNG: each pair of open-interval uniforms
+// (u1, u2) gives sqrt(-2 ln u1) * (cos 2 pi u2, sin 2 pi u2).
+void normal_fill(uint32_t seed, size_t n, float stddev, float *out) {
+    Rng rng(seed);
+    const double two_pi = 6.283185307179586476925286766559;
+    size_t i = 0;
+    while (i < n) {
+        const double u1 = rng.uniform_open();
+        const double u2 = rng.uniform_open();
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        out[i++] = (float)(stddev * r * std::cos(two_pi * u2));
+        if (i < n) out[i++] = (float)(stddev * r * std::sin(two_pi * u2));
+    }
+}
+
+}  // namespace lz

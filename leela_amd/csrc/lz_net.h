@@ -1,0 +1,69 @@
+// lz_net.h — layer tables, device weights and workspaces behind lz_ctx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace lz {
+
+struct Layer {
+    enum Kind { kConv = 0, kFC = 1 };
+    int kind = kConv;
+    int filter = 0;
+    int cin = 0;
+    int cout = 0;
+    std::vector<float> w;  // conv OIHW, FC [out][in]
+    std::vector<float> b;
+    // Device copies.
+    float *d_w32 = nullptr;  // FP32: trunk packed [cout/64][K][64]; head conv
+                             // OIHW; FC1 transposed [in][out]; FC2 as is
+    float *d_b = nullptr;
+};
+
+// A net as the reference pushes it (Network.cpp:206-233): a chain of convs
+// each followed by ELU whose last conv has one output channel, then either
+// nothing (policy: softmax head) or two inner products (value: FC+ELU, FC,
+// then (1 + tanh) / 2).
+struct Net {
+    std::vector<Layer> layers;
+    int side = 19;
+    int n_trunk = 0;    // convs before the 1-channel head conv
+    int head = -1;      // index of the 1-channel conv
+    int fc1 = -1, fc2 = -1;
+    int max_ch = 0;     // widest trunk activation
+};
+
+struct Workspace {
+    int cap = 0;             // positions the buffers hold
+    uint8_t *planes = nullptr;
+    float *act[2] = {nullptr, nullptr};
+    float *raw = nullptr;
+    float *out = nullptr;
+};
+
+int validate_net(Net &net, int which, std::string *msg);
+int upload_net(Net &net, int which, std::string *msg);
+void free_net(Net &net);
+int ensure_workspace(Workspace &ws, const Net nets[2], int n,
+                     std::string *msg);
+void free_workspace(Workspace &ws);
+// Records a HIP event on the launch stream before the first and after
+// every kernel launch of a forward.
+struct LaunchTimer {
+    std::vector<hipEvent_t> events;
+    hipError_t mark(hipStream_t s);
+    ~LaunchTimer();
+};
+
+int run_net(const Net &net, int which, int precision, int n,
+            const uint8_t *d_planes, float *d_raw, float *d_out,
+            float temperature, Workspace &ws, hipStream_t stream,
+            std::string *msg, LaunchTimer *timer = nullptr);
+
+void normal_fill(uint32_t seed, size_t n, float stddev, float *out);
+
+}  // namespace lz

@@ -1,0 +1,70 @@
+// lz_random.h — the reference's random number generator, restated for the
+// host side of this library: xoroshiro128+ with an LCG seed expansion
+// (Random.cpp:27-57, Random.h:81-110). Streams are bit-identical to
+// Random(seed) in the reference, which the position generator and the
+// synthetic weight sets rely on for reproducibility.
+#pragma once
+
+#include <cstdint>
+#include <limits>
+
+namespace lz {
+
+class Rng {
+public:
+    explicit Rng(uint32_t seed) { seedrandom(seed); }
+
+    // Random::seedrandom (Random.cpp:51-57).
+    void seedrandom(uint32_t seed) {
+        s_[0] = 741103597ull * (uint64_t)seed;
+        s_[1] = 741103597ull * s_[0];
+    }
+
+    // Random::random (Random.cpp:27-37).
+    uint64_t next() {
+        const uint64_t s0 = s_[0];
+        uint64_t s1 = s_[1];
+        const uint64_t r = s0 + s1;
+        s1 ^= s0;
+        s_[0] = rotl(s0, 55) ^ s1 ^ (s1 << 14);
+        s_[1] = rotl(s1, 36);
+        return r;
+    }
+
+    // Random::randuint16 / randuint32(max) / randuint32() (Random.cpp:39-49).
+    uint16_t randuint16(uint16_t max) {
+        return (uint16_t)(((next() >> 48) * max) >> 16);
+    }
+    uint32_t randuint32(uint32_t max) {
+        return (uint32_t)(((next() >> 32) * (uint64_t)max) >> 32);
+    }
+    uint32_t randuint32() { return (uint32_t)(next() >> 32); }
+
+    // Random::randfix<MAX> (Random.h:81-89, 107-110).
+    template <int MAX>
+    uint32_t randfix() {
+        static_assert(MAX > 0, "randfix range");
+        if (MAX == 2) return next() > (std::numeric_limits<uint64_t>::max() / 2);
+        return (uint32_t)(next() % MAX);
+    }
+
+    // Random::randflt (Random.cpp:59-65).
+    float randflt() {
+        const float umax = 1.0f / (float)(1u << 24);
+        return (float)(uint32_t)(next() >> 40) * umax;
+    }
+
+    // Uniform double in (0, 1) from the top 53 bits (never 0 or 1); used by
+    // the Box-Muller sampler of the synthetic weights.
+    double uniform_open() {
+        return ((double)(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    }
+
+private:
+    static uint64_t rotl(uint64_t x, int k) {
+        return (x << k) | (x >> (64 - k));
+    }
+    uint64_t s_[2];
+};
+
+}  // namespace lz
