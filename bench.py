@@ -73,7 +73,7 @@ def main() -> int:
 
     prec_name = args.precision
     if prec_name == "auto":
-        prec_name = "fp32"
+        prec_name = "bf16"
     prec = leela_amd.PRECISIONS[prec_name]
     B = args.batch
 
@@ -92,42 +92,60 @@ def main() -> int:
     d_vout = torch.empty((B, 1), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
 
-    def step():
-        be.forward_device(leela_amd.NET_POLICY, prec, B, d_planes.data_ptr(),
+    def step(p):
+        be.forward_device(leela_amd.NET_POLICY, p, B, d_planes.data_ptr(),
                           d_praw.data_ptr(), d_pout.data_ptr(), stream)
-        be.forward_device(leela_amd.NET_VALUE, prec, B, d_planes.data_ptr(),
+        be.forward_device(leela_amd.NET_VALUE, p, B, d_planes.data_ptr(),
                           d_vraw.data_ptr(), d_vout.data_ptr(), stream)
 
-    for _ in range(args.warmup):
-        step()
+    def timed(p, steps, warmup):
+        for _ in range(warmup):
+            step(p)
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(p)
+        torch.cuda.synchronize(dev)
+        return time.perf_counter() - t0
+
+    # FP32 plain form on the same positions and weights (the comparison
+    # the metric names), then the measured precision.
+    fp32_dt = timed(leela_amd.PREC_FP32, max(2, args.steps // 4), 1)
+    fp32_value = B * max(2, args.steps // 4) / fp32_dt
+    step(leela_amd.PREC_FP32)
     torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize(dev)
-    dt = time.perf_counter() - t0
+    fp32_logits = d_praw.cpu().numpy().copy()
+    fp32_win = d_vout.cpu().numpy().copy()
+
+    dt = timed(prec, args.steps, args.warmup)
     ms_per_step = dt * 1e3 / args.steps
     value = B * args.steps / dt
+    step(prec)
+    torch.cuda.synchronize(dev)
+    logits = d_praw.cpu().numpy()
+    win = d_vout.cpu().numpy()
+    top1 = float(np.mean(logits.argmax(1) == fp32_logits.argmax(1)))
+    dwin = float(np.abs(win - fp32_win).max())
 
-    # Dominant kernel: the trunk conv (3x3, 128->128) of the policy net,
-    # timed per launch with HIP events on the launch stream.
+    # Dominant kernel, timed per launch with HIP events on its stream.
     reps = 5
-    trunk_ms = []
+    kern_ms = []
     for _ in range(reps):
         ms = be.forward_timed(leela_amd.NET_POLICY, prec, B,
                               d_planes.data_ptr(), d_praw.data_ptr(),
                               d_pout.data_ptr(), stream)
         if prec_name == "fp32":
-            trunk_ms += list(ms[2:12])  # launches 2..11: conv3..conv12
+            kern_ms += list(ms[2:12])  # launches 2..11: conv3..conv12
         else:
-            trunk_ms += list(ms)
-    avg_ms = float(np.mean(trunk_ms))
+            kern_ms += list(ms)        # one fused launch per forward
+    avg_ms = float(np.mean(kern_ms))
     if prec_name == "fp32":
         kflops = 2.0 * 128 * 1152 * 361 * B  # one trunk layer, whole batch
         kernel = "conv_f32_kernel<3> (policy trunk 3x3 128->128)"
     else:
         kflops = float(nets.flops_per_position(nets.POLICY_SHAPES)) * B
-        kernel = "policy forward"
+        kernel = (f"fused_net_{prec_name} policy (13 convs + softmax, "
+                  f"{B} boards per launch)")
     achieved = kflops / (avg_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[prec_name]
 
@@ -158,6 +176,10 @@ def main() -> int:
             "weights": "He-normal seed 7 (set A), Network.cpp:82-137 shapes",
             "positions": "random legal play, seeds 1..B",
             "parallelism": "single GPU",
+            "fp32_plain_positions_per_s": round(fp32_value, 1),
+            "speedup_vs_fp32_plain": round(value / fp32_value, 3),
+            "top1_agreement_vs_fp32_plain": round(top1, 5),
+            "max_abs_value_diff_vs_fp32_plain": round(dwin, 6),
         },
         "roofline": {
             "bound": "mfma",

@@ -1,0 +1,39 @@
+// lz_fused.h — arguments of the persistent fused layer-chain kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lz {
+
+constexpr int kMaxFusedConvs = 16;
+
+// Device pointers of one net in the fused kernels' layouts. w[l] holds conv
+// l packed in MFMA fragment order: [k-step][m-tile][lane 0..63][8 values],
+// k-steps tap-major with 32 input channels each (see pack_conv_fragments).
+struct FusedArgs {
+    const void *w[kMaxFusedConvs];
+    const float *b[kMaxFusedConvs];
+    const float *fc1_wt = nullptr;  // value head: [361][hidden]
+    const float *fc1_b = nullptr;
+    const float *fc2_w = nullptr;
+    const float *fc2_b = nullptr;
+    int hidden = 0;
+    int nconv = 0;
+    int c = 0;   // trunk width
+    int m1 = 0;  // outputs of the 5x5 stem
+};
+
+// which: 0 policy (13 convs: 5x5 32->96, 3x3 96->128, 10x 128->128,
+// 128->1), 1 value (12 convs: 5x5 32->64, 10x 64->64, 64->1, + 2 FC).
+// grid <= 0 launches one workgroup per board.
+// Resident workgroups per CU of the fused kernel of net `which` (LDS and
+// register limited); the persistent grid is CUs times this.
+int fused_blocks_per_cu(int which);
+
+hipError_t fused_forward_bf16(int which, const uint8_t *planes,
+                              const FusedArgs &args, float *raw, float *out,
+                              int n, float temperature, int grid,
+                              hipStream_t stream);
+
+}  // namespace lz

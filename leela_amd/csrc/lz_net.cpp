@@ -3,6 +3,7 @@
 #include "lz_net.h"
 
 #include <cmath>
+#include <cstring>
 
 #include "leela_amd.h"
 #include "lz_internal.h"
@@ -118,14 +119,84 @@ int upload_net(Net &net, int which, std::string *msg) {
         *msg = std::string("weight upload: ") + hipGetErrorString(e);
         return LZ_ERR_HIP;
     }
+    // The fused kernels are specialised for the reference's two shape sets
+    // (Network.cpp:82-137): policy 5x5 32->96, 3x3 96->128, 10x 128->128,
+    // 128->1; value 5x5 32->64, 10x 64->64, 64->1, FC 361->H, FC H->1.
+    const auto &L = net.layers;
+    const int nconv = net.head + 1;
+    bool ok = nconv <= kMaxFusedConvs && L[0].filter == 5 && L[0].cin == 32;
+    const int c = nconv > 1 ? L[1].cout : 0;
+    for (int k = 1; ok && k < net.head; k++)
+        ok = L[k].filter == 3 && L[k].cout == c && (k == 1 || L[k].cin == c);
+    if (which == LZ_NET_POLICY)
+        ok = ok && nconv == 13 && c == 128 && L[0].cout == 96;
+    else
+        ok = ok && nconv == 12 && c == 64 && L[0].cout == 64;
+    net.fused_ok = ok;
+    if (!ok) return LZ_OK;
+    FusedArgs &A = net.fused;
+    for (int k = 0; k < nconv && e == hipSuccess; k++) {
+        e = upload(&net.layers[k].d_wbf16, pack_conv_fragments(L[k]));
+        A.w[k] = net.layers[k].d_wbf16;
+        A.b[k] = net.layers[k].d_b;
+    }
+    if (e != hipSuccess) {
+        *msg = std::string("BF16 weight upload: ") + hipGetErrorString(e);
+        return LZ_ERR_HIP;
+    }
+    A.nconv = nconv;
+    A.c = c;
+    A.m1 = L[0].cout;
+    if (which == LZ_NET_VALUE) {
+        A.fc1_wt = L[net.fc1].d_w32;
+        A.fc1_b = L[net.fc1].d_b;
+        A.fc2_w = L[net.fc2].d_w32;
+        A.fc2_b = L[net.fc2].d_b;
+        A.hidden = L[net.fc1].cout;
+    }
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    net.fused_grid = cus * fused_blocks_per_cu(which);
     return LZ_OK;
+}
+
+uint16_t float_to_bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u)  // inf / nan: truncate, keep nan
+        return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+std::vector<uint16_t> pack_conv_fragments(const Layer &L) {
+    const int F = L.filter, cin = L.cin, cout = L.cout;
+    const int CB = cin / 32, steps = F * F * CB, MT = (cout + 15) / 16;
+    std::vector<uint16_t> out((size_t)steps * MT * 64 * 8, 0);
+    size_t idx = 0;
+    for (int s = 0; s < steps; s++) {
+        const int tap = s / CB, cb = s % CB, dy = tap / F, dx = tap % F;
+        for (int m = 0; m < MT; m++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int e = 0; e < 8; e++, idx++) {
+                    const int o = m * 16 + (lane & 15);
+                    const int ch = cb * 32 + (lane >> 4) * 8 + e;
+                    if (o < cout)
+                        out[idx] = float_to_bf16_rne(
+                            L.w[(((size_t)o * cin + ch) * F + dy) * F + dx]);
+                }
+    }
+    return out;
 }
 
 void free_net(Net &net) {
     for (auto &L : net.layers) {
         dfree(L.d_w32);
         dfree(L.d_b);
+        dfree(L.d_wbf16);
         L.d_w32 = L.d_b = nullptr;
+        L.d_wbf16 = nullptr;
     }
 }
 
@@ -220,6 +291,22 @@ int run_net(const Net &net, int which, int precision, int n,
     if (precision == LZ_PREC_FP32)
         return run_fp32(net, which, n, d_planes, d_raw, d_out, temperature,
                         ws, stream, msg, timer);
+    if (precision == LZ_PREC_BF16) {
+        if (!net.fused_ok) {
+            *msg = "BF16 path implements only the reference layer shapes";
+            return LZ_ERR_UNSUPPORTED;
+        }
+        hipError_t e = timer ? timer->mark(stream) : hipSuccess;
+        if (e == hipSuccess)
+            e = fused_forward_bf16(which, d_planes, net.fused, d_raw, d_out, n,
+                                   temperature, net.fused_grid, stream);
+        if (e == hipSuccess && timer) e = timer->mark(stream);
+        if (e != hipSuccess) {
+            *msg = std::string("BF16 forward launch: ") + hipGetErrorString(e);
+            return LZ_ERR_HIP;
+        }
+        return LZ_OK;
+    }
     *msg = "precision not built yet";
     return LZ_ERR_UNSUPPORTED;
 }

@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "lz_fused.h"
+
 namespace lz {
 
 struct Layer {
@@ -22,6 +24,7 @@ struct Layer {
     float *d_w32 = nullptr;  // FP32: trunk packed [cout/64][K][64]; head conv
                              // OIHW; FC1 transposed [in][out]; FC2 as is
     float *d_b = nullptr;
+    uint16_t *d_wbf16 = nullptr;  // BF16 MFMA fragments (fused kernels)
 };
 
 // A net as the reference pushes it (Network.cpp:206-233): a chain of convs
@@ -35,6 +38,9 @@ struct Net {
     int head = -1;      // index of the 1-channel conv
     int fc1 = -1, fc2 = -1;
     int max_ch = 0;     // widest trunk activation
+    bool fused_ok = false;  // shapes the fused kernels implement
+    FusedArgs fused;        // device pointers for the BF16 fused kernel
+    int fused_grid = 0;     // persistent workgroups (CUs x occupancy)
 };
 
 struct Workspace {
@@ -63,6 +69,12 @@ int run_net(const Net &net, int which, int precision, int n,
             const uint8_t *d_planes, float *d_raw, float *d_out,
             float temperature, Workspace &ws, hipStream_t stream,
             std::string *msg, LaunchTimer *timer = nullptr);
+
+// Host packing of a conv into MFMA fragment order (BF16 bits, RNE):
+// [k-step][m-tile][lane][8], k-step s = tap * (cin/32) + channel block,
+// lane = row + 16 * g holding input channels 32*cb + 8*g .. +7.
+std::vector<uint16_t> pack_conv_fragments(const Layer &L);
+uint16_t float_to_bf16_rne(float f);
 
 void normal_fill(uint32_t seed, size_t n, float stddev, float *out);
 
