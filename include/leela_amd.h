@@ -102,6 +102,58 @@ int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
  * weight blobs (NN128.cpp, NNValue.cpp) are not available. */
 int lz_normal_fill(uint32_t seed, size_t n, float stddev, float *out);
 
+/* ---- Host-side input path (no GPU) ------------------------------------
+ * Positions, the synthetic-position generator and the plane encoders that
+ * feed lz_forward. They restate the reference's board engine and encoders
+ * (FastBoard / FullBoard / FastState, Network::gather_features_*), which
+ * stay on the caller's side of the OpenCL seam in the reference; they are
+ * exported so a batching front end and the tests can build inputs. */
+typedef struct lz_position lz_position;
+
+/* New empty side x side game (5..19), black to move. FastState::init_game
+ * (FastState.cpp:20-34). NULL on bad arguments. */
+lz_position *lz_position_new(int side, float komi);
+void lz_position_free(lz_position *pos);
+
+/* Plays `vertex` for the side to move (FastState::play_move, FastState.cpp:
+ * 360-389). Vertices use the reference numbering (y+1)*(side+2)+(x+1)
+ * (FastBoard::get_vertex, FastBoard.cpp:38-49); -1 passes. */
+int lz_position_play(lz_position *pos, int vertex);
+
+/* Resets and plays the synthetic-position generator: move count uniform in
+ * 0..max_moves from Random(seed) (Random.cpp:51-57), each move uniform over
+ * empty points that are not the ko square, not an own eye and not suicide
+ * (the filter of UCTNode::create_children, UCTNode.cpp:149-158). */
+int lz_position_random(lz_position *pos, uint32_t seed, int max_moves);
+
+/* info[7] = {side, to_move, komove, last_move, prev_last_move, movenum,
+ * passes} (FastState getters, FastState.cpp:391-576). */
+int lz_position_info(const lz_position *pos, int32_t *info);
+
+/* squares[y*side+x] = 0 black, 1 white, 2 empty (FastBoard::square_t). */
+int lz_position_squares(const lz_position *pos, uint8_t *squares);
+
+/* The 32 input planes of `net`, rotated by `symmetry` (0..7) as
+ * get_scored_moves_internal builds them (Network.cpp:765-773):
+ * planes uint8 [32][side*side]. Network::gather_features_policy /
+ * gather_features_value (Network.cpp:883-1201). */
+int lz_encode_planes(const lz_position *pos, int net, int symmetry,
+                     uint8_t *planes);
+
+/* Per-point board queries for the side to move, LZ_QUERY_FIELDS int32 per
+ * point (y*side+x), -1 where not applicable: {square, string liberties,
+ * string size, after_liberties own, after_liberties opponent, empty
+ * neighbours, saving_size, check_losing_ladder, check_winning_ladder,
+ * is_suicide, no_eye_fill, self_atari}. FastBoard.cpp:191-2837. */
+#define LZ_QUERY_FIELDS 12
+int lz_board_queries(const lz_position *pos, int32_t *out);
+
+/* Generator + encoder for n positions with seeds seed0..seed0+n-1:
+ * planes uint8 [n][32][side*side] of `net`, rotation 0. threads <= 0 uses
+ * every hardware thread. */
+int lz_make_batch(int n, int side, uint32_t seed0, int max_moves, int net,
+                  int threads, uint8_t *planes);
+
 /* Device name, as OpenCL::get_device_name (OpenCL.h:121). */
 const char *lz_device_name(lz_ctx *ctx);
 

@@ -221,31 +221,69 @@ __global__ __launch_bounds__(kThreads) void policy_head_f32_kernel(
     }
 }
 
-__global__ __launch_bounds__(kThreads) void value_head_f32_kernel(
+// Last 1-channel conv + ELU of the value net, written to h0 [n][S*S] for
+// value_fc_head.
+__global__ __launch_bounds__(kThreads) void value_headconv_f32_kernel(
     const float *__restrict__ in, const float *__restrict__ w,
-    const float *__restrict__ b, const float *__restrict__ fc1_wt,
-    const float *__restrict__ fc1_b, const float *__restrict__ fc2_w,
-    const float *__restrict__ fc2_b, float *__restrict__ raw,
-    float *__restrict__ out, int cin, int S, int hidden) {
+    const float *__restrict__ b, float *__restrict__ h0, int cin, int S) {
     __shared__ float s_in[kHeadCK * kHeadSP * kHeadSP];
     __shared__ float s_w[kHeadCK * 9];
     __shared__ float s_h0[kMaxSide * kMaxSide];
-    __shared__ float s_red[kThreads / 64];
     const int board = blockIdx.x, tid = threadIdx.x, P = S * S;
     head_conv3(in + (size_t)board * cin * P, w, b[0], cin, S, s_in, s_w,
                s_h0);
-    // innerproduct<361, 256> with ELU, then innerproduct<256, 1> without
-    // (Network.cpp:395-423, 733-734).
-    float part = 0.0f;
-    for (int o = tid; o < hidden; o += kThreads) {
-        float acc = 0.0f;
-        for (int i = 0; i < P; i++) acc += fc1_wt[(size_t)i * hidden + o] * s_h0[i];
-        part += fc2_w[o] * elu(acc + fc1_b[o]);
+    for (int p = tid; p < P; p += kThreads) h0[(size_t)board * P + p] = s_h0[p];
+}
+
+// Batched inner-product head of the value net (Network.cpp:733-736):
+// hidden = ELU(W1 h0 + b1), v = w2 . hidden + b2, out = (1 + tanh v) / 2.
+// A workgroup takes kFcBoards boards so every W1 element loaded from L2 is
+// used kFcBoards times; thread o accumulates output o for all of them.
+constexpr int kFcBoards = 16;
+
+__global__ __launch_bounds__(kThreads) void value_fc_kernel(
+    const float *__restrict__ h0, const float *__restrict__ w1t,
+    const float *__restrict__ b1, const float *__restrict__ w2,
+    const float *__restrict__ b2, float *__restrict__ raw,
+    float *__restrict__ out, int n, int P, int H) {
+    __shared__ float s_h[kFcBoards][kMaxSide * kMaxSide];
+    __shared__ float s_red[kFcBoards][kThreads / 64];
+    const int b0 = blockIdx.x * kFcBoards, tid = threadIdx.x;
+    const int nb = min(kFcBoards, n - b0);
+    for (int i = tid; i < kFcBoards * P; i += kThreads) {
+        const int bb = i / P, p = i % P;
+        s_h[bb][p] = bb < nb ? h0[(size_t)(b0 + bb) * P + p] : 0.0f;
     }
-    const float v = block_reduce(part, false, s_red) + fc2_b[0];
-    if (tid == 0) {
-        if (raw) raw[board] = v;
-        if (out) out[board] = (1.0f + tanhf(v)) / 2.0f;
+    __syncthreads();
+    float part[kFcBoards];
+#pragma unroll
+    for (int bb = 0; bb < kFcBoards; bb++) part[bb] = 0.0f;
+    for (int o = tid; o < H; o += kThreads) {
+        float acc[kFcBoards];
+#pragma unroll
+        for (int bb = 0; bb < kFcBoards; bb++) acc[bb] = 0.0f;
+        for (int i = 0; i < P; i++) {
+            const float wv = w1t[(size_t)i * H + o];
+#pragma unroll
+            for (int bb = 0; bb < kFcBoards; bb++) acc[bb] += wv * s_h[bb][i];
+        }
+        const float bo = b1[o], wo = w2[o];
+#pragma unroll
+        for (int bb = 0; bb < kFcBoards; bb++) part[bb] += wo * elu(acc[bb] + bo);
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int bb = 0; bb < kFcBoards; bb++) {
+        float v = part[bb];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) s_red[bb][wave] = v;
+    }
+    __syncthreads();
+    if (tid < nb) {
+        float v = b2[0];
+        for (int w = 0; w < kThreads / 64; w++) v += s_red[tid][w];
+        if (raw) raw[b0 + tid] = v;
+        if (out) out[b0 + tid] = (1.0f + tanhf(v)) / 2.0f;
     }
 }
 
@@ -290,16 +328,27 @@ hipError_t policy_head_f32(const float *in, const float *w, const float *b,
     return hipGetLastError();
 }
 
-hipError_t value_head_f32(const float *in, const float *w, const float *b,
-                          const float *fc1_wt, const float *fc1_b,
-                          const float *fc2_w, const float *fc2_b, float *raw,
-                          float *out, int n, int cin, int S, int hidden,
-                          hipStream_t stream) {
+hipError_t value_headconv_f32(const float *in, const float *w,
+                              const float *b, float *h0, int n, int cin,
+                              int S, hipStream_t stream) {
     if (S < 1 || S > kMaxSide || n <= 0 || cin % kHeadCK != 0)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(value_head_f32_kernel, dim3(n), dim3(kThreads), 0,
-                       stream, in, w, b, fc1_wt, fc1_b, fc2_w, fc2_b, raw,
-                       out, cin, S, hidden);
+    hipLaunchKernelGGL(value_headconv_f32_kernel, dim3(n), dim3(kThreads), 0,
+                       stream, in, w, b, h0, cin, S);
+    return hipGetLastError();
+}
+
+hipError_t value_fc_head(const float *h0, const float *fc1_wt,
+                         const float *fc1_b, const float *fc2_w,
+                         const float *fc2_b, float *raw, float *out, int n,
+                         int P, int hidden, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    if (P < 1 || P > kMaxSide * kMaxSide || hidden < 1)
+        return hipErrorInvalidValue;
+    const int grid = (n + kFcBoards - 1) / kFcBoards;
+    hipLaunchKernelGGL(value_fc_kernel, dim3(grid), dim3(kThreads), 0, stream,
+                       h0, fc1_wt, fc1_b, fc2_w, fc2_b, raw, out, n, P,
+                       hidden);
     return hipGetLastError();
 }
 

@@ -1,29 +1,33 @@
-// fused_bf16.hip — persistent fused layer-chain kernel: the whole policy
-// (or value) net of one board per workgroup pass, BF16 on MFMA with FP32
-// accumulation, activations resident in LDS.
+// fused_bf16.hip — persistent fused layer-chain kernel: all convs of the
+// policy (or value) net for one board per workgroup pass, BF16 on MFMA with
+// FP32 accumulation, activations resident in LDS.
 //
 // Work decomposition
-//   * A workgroup (4 waves) owns one board at a time and loops over boards
+//   * A workgroup owns one board at a time and loops over boards
 //     (persistent grid). The board's activations live in LDS as BF16 in an
 //     HWC tile with a zero halo: [SP][SP][PS], SP = 19 + 2*pad, PS = C + 8
 //     (the 8-element pad staggers pixels across LDS banks).
 //   * Each conv is an implicit GEMM D[cout x 361] = W[cout x K] B[K x 361],
 //     K = k*k*cin ordered tap-major, 32 channels per MFMA k-step
-//     (v_mfma_f32_16x16x32_bf16). A wave owns all output channels of six
-//     16-pixel column tiles (4 waves x 6 x 16 = 384 >= 361), so every
-//     B fragment is read from LDS exactly once per layer: one 16-byte
-//     ds_read per lane at pixel_base + tap_offset + channel_block.
+//     (v_mfma_f32_16x16x32_bf16). The waves form a WM (output channels) x 4
+//     (pixels) grid (policy 2 x 4, value 1 x 4); a wave owns 1/WM of the
+//     output channels of six 16-pixel column tiles (4 x 6 x 16 = 384 >= 361
+//     pixels). Its B fragments are
+//     one 16-byte ds_read per lane at pixel_base + tap_offset +
+//     channel_block, prefetched one k-step ahead.
 //   * Weights are pre-packed on the host in MFMA fragment order, [k-step]
-//     [m-tile][lane][8], so the A operand is one coalesced 16-byte global
-//     load per lane, prefetched one k-step ahead. All waves of a workgroup
-//     walk the same k-steps together, so the 4 reads of each fragment hit
-//     in L1 and each CU pulls every layer's weights from L2 once per board.
+//     [m-tile][lane][8], so an A operand is one coalesced 16-byte global load
+//     per lane, prefetched one k-step ahead. The four pixel-waves of a
+//     channel half walk the same k-steps together, so repeated fragment
+//     reads hit in L1 and each CU pulls a layer's weights from L2 once per
+//     board.
 //   * Epilogue: bias + ELU in FP32 (Network.cpp:382-392), rounded to BF16,
-//     written back over the input tile after a barrier. The last
-//     1-channel conv keeps FP32 logits for softmax(T) (Network.cpp:450-469)
-//     or feeds the FP32 value head (ip13 + ELU, ip14, (1 + tanh) / 2,
-//     Network.cpp:733-736).
+//     written back over the input tile after a barrier. The last 1-channel
+//     conv keeps FP32: softmax(T) over the logits (Network.cpp:450-469) for
+//     the policy net; for the value net the ELU'd plane goes to HBM and the
+//     batched inner-product head (value_fc_head) finishes it.
 #include "lz_fused.h"
+#include "lz_internal.h"
 
 namespace lz {
 namespace {
@@ -33,11 +37,10 @@ typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kThreads = 256;
-constexpr int kWaves = 4;
-constexpr int kNT = 6;  // 16-pixel tiles per wave
+constexpr int kWN = 4;                // waves along pixels
+constexpr int kNT = 6;                // 16-pixel tiles per wave
 constexpr int kP = 361;
-constexpr int kInPS = 40;  // pixel stride of the 32-plane input tile
+constexpr int kInPS = 40;             // pixel stride of the 32-plane input
 
 __device__ __forceinline__ float elu(float v) {
     return v > 0.0f ? v : (expf(v) - 1.0f);
@@ -51,79 +54,86 @@ __device__ __forceinline__ int pix_base(int j) {
     return ((j / 19) * SP + (j % 19)) * PS;
 }
 
+__device__ __forceinline__ const bf16x8 *wfrag(const FusedArgs &a, int l) {
+    return static_cast<const bf16x8 *>(a.w[l]);
+}
+
 // One same-padded conv of the tile `act` (side SP, stride PS, CIN input
-// channels) into acc[MT][kNT]: the wave's column tiles, all MT*16 rows.
-template <int F, int CIN, int MT, int SP, int PS>
+// channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the packed
+// weights) for the wave's column tiles.
+template <int F, int CIN, int MT, int MTW, int SP, int PS>
 __device__ __forceinline__ void conv_tile(const bf16 *act,
                                           const bf16x8 *__restrict__ wpk,
-                                          f32x4 (&acc)[MT][kNT],
-                                          const int (&pb)[kNT], int lane) {
+                                          f32x4 (&acc)[MTW][kNT],
+                                          const int (&pb)[kNT], int mt0,
+                                          int lane) {
     constexpr int CB = CIN / 32;
     constexpr int STEPS = F * F * CB;
     const int g8 = (lane >> 4) * 8;
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+    for (int m = 0; m < MTW; m++)
 #pragma unroll
         for (int t = 0; t < kNT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // One A register per m-tile: refilled with the next k-step's fragment
-    // as soon as the current step's MFMAs on it have issued.
-    bf16x8 a[MT];
+    const bf16x8 *wl = wpk + mt0 * 64 + lane;
+    bf16x8 a[MTW], b[kNT];
 #pragma unroll
-    for (int m = 0; m < MT; m++) a[m] = wpk[m * 64 + lane];
+    for (int m = 0; m < MTW; m++) a[m] = wl[m * 64];
+#pragma unroll
+    for (int t = 0; t < kNT; t++)
+        b[t] = *reinterpret_cast<const bf16x8 *>(act + pb[t] + g8);
 
     for (int s = 0; s < STEPS; s++) {
-        const int tap = s / CB, cb = s % CB;
+        const int nxt = s + 1 < STEPS ? s + 1 : s;
+        const int tap = nxt / CB, cb = nxt % CB;
         const int off = ((tap / F) * SP + (tap % F)) * PS + cb * 32 + g8;
-        bf16x8 b[kNT];
+        bf16x8 bn[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++)
-            b[t] = *reinterpret_cast<const bf16x8 *>(act + pb[t] + off);
-        const int nxt = s + 1 < STEPS ? s + 1 : s;
+            bn[t] = *reinterpret_cast<const bf16x8 *>(act + pb[t] + off);
 #pragma unroll
-        for (int m = 0; m < MT; m++) {
+        for (int m = 0; m < MTW; m++) {
 #pragma unroll
             for (int t = 0; t < kNT; t++)
                 acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                     a[m], b[t], acc[m][t], 0, 0, 0);
-            a[m] = wpk[(nxt * MT + m) * 64 + lane];
+            a[m] = wl[(nxt * MT + m) * 64];
         }
+#pragma unroll
+        for (int t = 0; t < kNT; t++) b[t] = bn[t];
     }
 }
 
-// Bias + ELU of acc, written as BF16 into the output tile (side 21, stride
+// Bias + ELU of acc, written as BF16 into the trunk tile (side 21, stride
 // PSO). D layout of 16x16 MFMA: lane holds rows 4*(lane/16)+r of column
 // lane%16, i.e. four consecutive output channels of one pixel.
-template <int MT, int PSO>
-__device__ __forceinline__ void store_tile(bf16 *act, const f32x4 (&acc)[MT][kNT],
+template <int MTW, int PSO>
+__device__ __forceinline__ void store_tile(bf16 *act,
+                                           const f32x4 (&acc)[MTW][kNT],
                                            const float *__restrict__ bias,
-                                           int cout, int wave, int lane) {
+                                           int cout, int mt0, int nt0,
+                                           int lane) {
     const int g4 = (lane >> 4) * 4;
 #pragma unroll
-    for (int m = 0; m < MT; m++) {
+    for (int m = 0; m < MTW; m++) {
+        const int o0 = (mt0 + m) * 16 + g4;
         float bv[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int o = m * 16 + g4 + r;
-            bv[r] = o < cout ? bias[o] : 0.0f;
-        }
+        for (int r = 0; r < 4; r++) bv[r] = o0 + r < cout ? bias[o0 + r] : 0.0f;
 #pragma unroll
         for (int t = 0; t < kNT; t++) {
-            const int j = (wave * kNT + t) * 16 + (lane & 15);
+            const int j = (nt0 + t) * 16 + (lane & 15);
             if (j >= kP) continue;
             bf16x4 v;
 #pragma unroll
             for (int r = 0; r < 4; r++) v[r] = (bf16)elu(acc[m][t][r] + bv[r]);
             const int y = j / 19 + 1, x = j % 19 + 1;
-            *reinterpret_cast<bf16x4 *>(act + (y * 21 + x) * PSO + m * 16 + g4) = v;
+            *reinterpret_cast<bf16x4 *>(act + (y * 21 + x) * PSO + o0) = v;
         }
     }
 }
 
-__device__ __forceinline__ const bf16x8 *wfrag(const FusedArgs &a, int l) {
-    return static_cast<const bf16x8 *>(a.w[l]);
-}
-
+template <int kWaves>
 __device__ float block_sum(float v, float *s_red) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __syncthreads();
@@ -134,6 +144,7 @@ __device__ float block_sum(float v, float *s_red) {
     return r;
 }
 
+template <int kWaves>
 __device__ float block_max(float v, float *s_red) {
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
     __syncthreads();
@@ -146,29 +157,39 @@ __device__ float block_max(float v, float *s_red) {
 
 // C: trunk width (128 policy, 64 value); M1: outputs of the 5x5 stem;
 // NMID: 3x3 convs between stem and head (the first takes M1 inputs).
-template <int C, int M1, int NMID, bool VALUE>
-__global__ __launch_bounds__(kThreads, 1) void fused_net_bf16(
-    const uint8_t *__restrict__ planes, FusedArgs args, float *__restrict__ raw,
-    float *__restrict__ out, int n, float temperature) {
+// Policy: writes raw = ELU'd logits and out = softmax(raw / T).
+// Value: writes the ELU'd head plane to h0 [n][361] for value_fc_head.
+// kWM waves split the output channels, kWN = 4 the pixels; OCC is the
+// waves per SIMD the register budget is sized for.
+template <int C, int M1, int NMID, bool VALUE, int kWM, int OCC>
+__global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net_bf16(
+    const uint8_t *__restrict__ planes, FusedArgs args,
+    float *__restrict__ raw, float *__restrict__ out, int n,
+    float temperature) {
     constexpr int PS = C + 8;
     constexpr int SP = 21;
     constexpr int TILE = SP * SP * PS;         // trunk tile (bf16 elements)
     constexpr int TILE0 = 23 * 23 * kInPS;     // stem input tile
     constexpr int TILE_MAX = TILE > TILE0 ? TILE : TILE0;
-    constexpr int MT = C / 16;
-    constexpr int MT1 = (M1 + 15) / 16;
-    static_assert(C % 32 == 0 && M1 % 32 == 0, "channel blocking");
+    constexpr int kWaves = kWM * kWN;
+    constexpr int kThreads = kWaves * 64;
+    constexpr int MT = C / 16, MTW = MT / kWM;
+    constexpr int MT1 = M1 / 16, MTW1 = MT1 / kWM;
+    static_assert(C % 32 == 0 && M1 % 32 == 0 && MT % kWM == 0 &&
+                      MT1 % kWM == 0, "channel blocking");
 
     __shared__ __attribute__((aligned(16))) bf16 s_act[TILE_MAX];
     __shared__ float s_vec[384];
     __shared__ float s_red[kWaves];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / kWN, wn = wave % kWN;
+    const int nt0 = wn * kNT;
 
     int pb_in[kNT], pb[kNT];
 #pragma unroll
     for (int t = 0; t < kNT; t++) {
-        const int j = (wave * kNT + t) * 16 + (lane & 15);
+        const int j = (nt0 + t) * 16 + (lane & 15);
         pb_in[t] = pix_base<23, kInPS>(j);
         pb[t] = pix_base<SP, PS>(j);
     }
@@ -179,8 +200,8 @@ __global__ __launch_bounds__(kThreads, 1) void fused_net_bf16(
         for (int i = tid; i < 23 * 23 * 4; i += kThreads) {
             const int pp = i >> 2, cg = (i & 3) * 8;
             const int y = pp / 23 - 2, x = pp % 23 - 2;
-            bf16x8 v;
             const bool in = y >= 0 && y < 19 && x >= 0 && x < 19;
+            bf16x8 v;
 #pragma unroll
             for (int e = 0; e < 8; e++)
                 v[e] = (bf16)(in ? (float)src[(cg + e) * kP + y * 19 + x] : 0.0f);
@@ -190,77 +211,84 @@ __global__ __launch_bounds__(kThreads, 1) void fused_net_bf16(
 
         // Stem: 5x5, 32 -> M1.
         {
-            f32x4 acc[MT1][kNT];
-            conv_tile<5, 32, MT1, 23, kInPS>(s_act, wfrag(args, 0), acc, pb_in, lane);
+            f32x4 acc[MTW1][kNT];
+            conv_tile<5, 32, MT1, MTW1, 23, kInPS>(s_act, wfrag(args, 0), acc,
+                                                   pb_in, wm * MTW1, lane);
             __syncthreads();
             // Switch to the trunk geometry: clear tile and halo, then write.
             for (int i = tid * 8; i < TILE; i += kThreads * 8)
                 *reinterpret_cast<bf16x8 *>(s_act + i) = bf16x8{};
             __syncthreads();
-            store_tile<MT1, PS>(s_act, acc, args.b[0], M1, wave, lane);
+            store_tile<MTW1, PS>(s_act, acc, args.b[0], M1, wm * MTW1, nt0,
+                                 lane);
             __syncthreads();
         }
         // Middle 3x3 convs; the first reads M1 channels.
         {
-            f32x4 acc[MT][kNT];
-            conv_tile<3, M1, MT, SP, PS>(s_act, wfrag(args, 1), acc, pb, lane);
+            f32x4 acc[MTW][kNT];
+            conv_tile<3, M1, MT, MTW, SP, PS>(s_act, wfrag(args, 1), acc, pb,
+                                              wm * MTW, lane);
             __syncthreads();
-            store_tile<MT, PS>(s_act, acc, args.b[1], C, wave, lane);
+            store_tile<MTW, PS>(s_act, acc, args.b[1], C, wm * MTW, nt0, lane);
             __syncthreads();
         }
         for (int l = 2; l <= NMID; l++) {
-            f32x4 acc[MT][kNT];
-            conv_tile<3, C, MT, SP, PS>(s_act, wfrag(args, l), acc, pb, lane);
+            f32x4 acc[MTW][kNT];
+            conv_tile<3, C, MT, MTW, SP, PS>(s_act, wfrag(args, l), acc, pb,
+                                             wm * MTW, lane);
             __syncthreads();
-            store_tile<MT, PS>(s_act, acc, args.b[l], C, wave, lane);
+            store_tile<MTW, PS>(s_act, acc, args.b[l], C, wm * MTW, nt0, lane);
             __syncthreads();
         }
-        // Head: 3x3, C -> 1, kept in FP32.
-        {
+        // Head: 3x3, C -> 1, in FP32; the channel-half-0 waves cover all
+        // pixels.
+        if (wm == 0) {
             f32x4 acc[1][kNT];
-            conv_tile<3, C, 1, SP, PS>(s_act, wfrag(args, NMID + 1), acc, pb, lane);
+            conv_tile<3, C, 1, 1, SP, PS>(s_act, wfrag(args, NMID + 1), acc,
+                                          pb, 0, lane);
             const float hb = args.b[NMID + 1][0];
             if (lane < 16) {
 #pragma unroll
                 for (int t = 0; t < kNT; t++) {
-                    const int j = (wave * kNT + t) * 16 + lane;
+                    const int j = (nt0 + t) * 16 + lane;
                     if (j < kP) s_vec[j] = elu(acc[0][t][0] + hb);
                 }
             }
-            __syncthreads();
         }
-        if (!VALUE) {
+        __syncthreads();
+        if (VALUE) {
+            for (int p = tid; p < kP; p += kThreads)
+                args.h0[(size_t)board * kP + p] = s_vec[p];
+        } else {
             float mx = -INFINITY;
             for (int p = tid; p < kP; p += kThreads) mx = fmaxf(mx, s_vec[p]);
-            mx = block_max(mx, s_red);
+            mx = block_max<kWaves>(mx, s_red);
             const float alpha = mx / temperature;
             float e[2] = {0.f, 0.f}, sum = 0.f;
             for (int i = 0, p = tid; p < kP; p += kThreads, i++) {
                 e[i] = expf(s_vec[p] / temperature - alpha);
                 sum += e[i];
             }
-            sum = block_sum(sum, s_red);
+            sum = block_sum<kWaves>(sum, s_red);
             for (int i = 0, p = tid; p < kP; p += kThreads, i++) {
                 if (raw) raw[(size_t)board * kP + p] = s_vec[p];
                 if (out) out[(size_t)board * kP + p] = e[i] / sum;
-            }
-        } else {
-            float part = 0.0f;
-            for (int o = tid; o < args.hidden; o += kThreads) {
-                float a = 0.0f;
-                for (int i = 0; i < kP; i++)
-                    a += args.fc1_wt[(size_t)i * args.hidden + o] * s_vec[i];
-                part += args.fc2_w[o] * elu(a + args.fc1_b[o]);
-            }
-            const float v = block_sum(part, s_red) + args.fc2_b[0];
-            if (tid == 0) {
-                if (raw) raw[board] = v;
-                if (out) out[board] = (1.0f + tanhf(v)) / 2.0f;
             }
         }
         __syncthreads();
     }
 }
+
+// Grid occupancy: the policy tile (121 KB LDS) allows one workgroup per
+// CU, the value tile (65 KB) two.
+// Policy: one 8-wave workgroup per CU (121 KB LDS tile), 2 waves per SIMD.
+// Value: 4-wave workgroups, two per CU (65 KB tile each), 2 waves per SIMD.
+constexpr int kPolicyWM = 2, kValueWM = 1;
+constexpr int kPolicyThreads = kPolicyWM * kWN * 64;
+constexpr int kValueThreads = kValueWM * kWN * 64;
+
+#define POLICY_KERNEL fused_net_bf16<128, 96, 11, false, kPolicyWM, 2>
+#define VALUE_KERNEL fused_net_bf16<64, 64, 10, true, kValueWM, 2>
 
 }  // namespace
 
@@ -269,9 +297,9 @@ int fused_blocks_per_cu(int which) {
     hipError_t e =
         which == 0
             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, fused_net_bf16<128, 96, 11, false>, kThreads, 0)
+                  &blocks, POLICY_KERNEL, kPolicyThreads, 0)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, fused_net_bf16<64, 64, 10, true>, kThreads, 0);
+                  &blocks, VALUE_KERNEL, kValueThreads, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
@@ -284,17 +312,18 @@ hipError_t fused_forward_bf16(int which, const uint8_t *planes,
     if (which == 0) {
         if (args.nconv != 13 || args.c != 128 || args.m1 != 96)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL((fused_net_bf16<128, 96, 11, false>), dim3(grid),
-                           dim3(kThreads), 0, stream, planes, args, raw, out,
-                           n, temperature);
-    } else {
-        if (args.nconv != 12 || args.c != 64 || args.m1 != 64)
-            return hipErrorInvalidValue;
-        hipLaunchKernelGGL((fused_net_bf16<64, 64, 10, true>), dim3(grid),
-                           dim3(kThreads), 0, stream, planes, args, raw, out,
-                           n, temperature);
+        hipLaunchKernelGGL(POLICY_KERNEL, dim3(grid), dim3(kPolicyThreads), 0,
+                           stream, planes, args, raw, out, n, temperature);
+        return hipGetLastError();
     }
-    return hipGetLastError();
+    if (args.nconv != 12 || args.c != 64 || args.m1 != 64 || !args.h0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(VALUE_KERNEL, dim3(grid), dim3(kValueThreads), 0, stream,
+                       planes, args, raw, out, n, temperature);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return value_fc_head(args.h0, args.fc1_wt, args.fc1_b, args.fc2_w,
+                         args.fc2_b, raw, out, n, kP, args.hidden, stream);
 }
 
 }  // namespace lz

@@ -18,6 +18,7 @@ struct FusedArgs {
     const float *fc1_b = nullptr;
     const float *fc2_w = nullptr;
     const float *fc2_b = nullptr;
+    float *h0 = nullptr;            // value head plane [n][361], per call
     int hidden = 0;
     int nconv = 0;
     int c = 0;   // trunk width

@@ -1,0 +1,149 @@
+// lz_host_abi.cpp — C ABI of the host-side input path: game positions, the
+// position generator, the feature encoders and the board queries behind
+// them. No GPU work happens here.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "input_planes.h"
+#include "go_board.h"
+#include "leela_amd.h"
+
+struct lz_position {
+    lz::go::GameState st;
+};
+
+namespace {
+
+bool valid_vertex(const lz::go::Board &b, int v) {
+    if (v < 0 || v >= b.width() * b.width()) return false;
+    const auto xy = b.xy(v);
+    return xy.first >= 0 && xy.first < b.side() && xy.second >= 0 &&
+           xy.second < b.side();
+}
+
+}  // namespace
+
+extern "C" {
+
+lz_position *lz_position_new(int side, float komi) {
+    if (side < 5 || side > lz::go::kMaxSide) return nullptr;
+    lz_position *p = new (std::nothrow) lz_position();
+    if (p) p->st.init(side, komi);
+    return p;
+}
+
+void lz_position_free(lz_position *pos) { delete pos; }
+
+int lz_position_play(lz_position *pos, int vertex) {
+    if (!pos) return LZ_ERR_ARG;
+    const auto &b = pos->st.board;
+    if (vertex != lz::go::kPass) {
+        if (!valid_vertex(b, vertex) || b.at(vertex) != lz::go::EMPTY)
+            return LZ_ERR_ARG;
+    }
+    pos->st.play(vertex);
+    return LZ_OK;
+}
+
+int lz_position_random(lz_position *pos, uint32_t seed, int max_moves) {
+    if (!pos || max_moves < 0) return LZ_ERR_ARG;
+    const int side = pos->st.board.side();
+    lz::go::random_position(pos->st, side, seed, max_moves);
+    return LZ_OK;
+}
+
+int lz_position_info(const lz_position *pos, int32_t *info) {
+    if (!pos || !info) return LZ_ERR_ARG;
+    const auto &s = pos->st;
+    info[0] = s.board.side();
+    info[1] = s.board.to_move();
+    info[2] = s.komove;
+    info[3] = s.last_move;
+    info[4] = s.prev_last;
+    info[5] = s.movenum;
+    info[6] = s.passes;
+    return LZ_OK;
+}
+
+int lz_position_squares(const lz_position *pos, uint8_t *squares) {
+    if (!pos || !squares) return LZ_ERR_ARG;
+    const auto &b = pos->st.board;
+    for (int y = 0; y < b.side(); y++)
+        for (int x = 0; x < b.side(); x++)
+            squares[y * b.side() + x] = b.at(b.vertex(x, y));
+    return LZ_OK;
+}
+
+int lz_encode_planes(const lz_position *pos, int net, int symmetry,
+                     uint8_t *planes) {
+    if (!pos || !planes || symmetry < 0 || symmetry > 7) return LZ_ERR_ARG;
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE) return LZ_ERR_ARG;
+    const int S = pos->st.board.side();
+    std::vector<uint8_t> tmp((size_t)lz::kPlanes * S * S);
+    if (net == LZ_NET_POLICY)
+        lz::gather_features_policy(pos->st, tmp.data());
+    else
+        lz::gather_features_value(pos->st, tmp.data());
+    lz::rotate_planes(tmp.data(), symmetry, S, planes);
+    return LZ_OK;
+}
+
+int lz_board_queries(const lz_position *pos, int32_t *out) {
+    if (!pos || !out) return LZ_ERR_ARG;
+    const auto &b = pos->st.board;
+    const int S = b.side(), tm = b.to_move();
+    for (int y = 0; y < S; y++)
+        for (int x = 0; x < S; x++) {
+            const int v = b.vertex(x, y);
+            int32_t *q = out + (size_t)(y * S + x) * LZ_QUERY_FIELDS;
+            std::fill(q, q + LZ_QUERY_FIELDS, -1);
+            q[0] = b.at(v);
+            if (b.at(v) != lz::go::EMPTY) {
+                q[1] = b.string_liberties(v);
+                q[2] = b.string_size(v);
+                continue;
+            }
+            const auto al = b.after_liberties(tm, v);
+            q[3] = al.first;
+            q[4] = al.second;
+            q[5] = b.empty_neighbours(v);
+            q[6] = b.saving_size(tm, v);
+            // Read only where the encoders read it (Network.cpp:1015-1022).
+            q[7] = (q[5] == 2 && q[6] > 0 && b.check_losing_ladder(tm, v)) ? 1 : 0;
+            q[8] = b.check_winning_ladder(tm, v) ? 1 : 0;
+            q[9] = b.is_suicide(v, tm) ? 1 : 0;
+            q[10] = b.no_eye_fill(v) ? 1 : 0;
+            q[11] = b.self_atari(tm, v) ? 1 : 0;
+        }
+    return LZ_OK;
+}
+
+int lz_make_batch(int n, int side, uint32_t seed0, int max_moves, int net,
+                  int threads, uint8_t *planes) {
+    if (n < 0 || !planes || side < 5 || side > lz::go::kMaxSide ||
+        max_moves < 0 || (net != LZ_NET_POLICY && net != LZ_NET_VALUE))
+        return LZ_ERR_ARG;
+    const size_t per = (size_t)lz::kPlanes * side * side;
+    if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+    threads = std::max(1, std::min(threads, std::max(n, 1)));
+    auto work = [&](int t) {
+        lz::go::GameState st;
+        for (int i = t; i < n; i += threads) {
+            lz::go::random_position(st, side, seed0 + (uint32_t)i, max_moves);
+            if (net == LZ_NET_POLICY)
+                lz::gather_features_policy(st, planes + per * i);
+            else
+                lz::gather_features_value(st, planes + per * i);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    return LZ_OK;
+}
+
+}  // extern "C"

@@ -25,12 +25,17 @@ hipError_t policy_head_f32(const float *in, const float *w, const float *b,
                            float *raw, float *out, int n, int cin, int S,
                            float temperature, hipStream_t stream);
 
-// Last 1-channel conv + ELU, FC (S*S -> hidden) + ELU, FC (hidden -> 1),
-// (1 + tanh) / 2. fc1_wt is the first FC transposed: [S*S][hidden].
-hipError_t value_head_f32(const float *in, const float *w, const float *b,
-                          const float *fc1_wt, const float *fc1_b,
-                          const float *fc2_w, const float *fc2_b, float *raw,
-                          float *out, int n, int cin, int S, int hidden,
-                          hipStream_t stream);
+// Last 1-channel conv + ELU of the value net into h0 [n][S*S].
+hipError_t value_headconv_f32(const float *in, const float *w, const float *b,
+                              float *h0, int n, int cin, int S,
+                              hipStream_t stream);
+
+// Value inner-product head over a batch: FC (P -> hidden) + ELU, FC
+// (hidden -> 1), (1 + tanh) / 2. fc1_wt is the first FC transposed:
+// [P][hidden]. Shared by every precision (the FCs stay FP32).
+hipError_t value_fc_head(const float *h0, const float *fc1_wt,
+                         const float *fc1_b, const float *fc2_w,
+                         const float *fc2_b, float *raw, float *out, int n,
+                         int P, int hidden, hipStream_t stream);
 
 }  // namespace lz

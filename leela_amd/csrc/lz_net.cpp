@@ -217,6 +217,7 @@ int ensure_workspace(Workspace &ws, const Net nets[2], int n,
     if (e == hipSuccess) e = dalloc(&ws.act[1], (size_t)n * ch * P);
     if (e == hipSuccess) e = dalloc(&ws.raw, (size_t)n * P);
     if (e == hipSuccess) e = dalloc(&ws.out, (size_t)n * P);
+    if (e == hipSuccess) e = dalloc(&ws.h0, (size_t)n * P);
     if (e != hipSuccess) {
         free_workspace(ws);
         *msg = std::string("workspace allocation: ") + hipGetErrorString(e);
@@ -232,6 +233,7 @@ void free_workspace(Workspace &ws) {
     dfree(ws.act[1]);
     dfree(ws.raw);
     dfree(ws.out);
+    dfree(ws.h0);
     ws = Workspace();
 }
 
@@ -271,9 +273,11 @@ static int run_fp32(const Net &net, int which, int n, const uint8_t *planes,
         } else {
             const Layer &F1 = net.layers[net.fc1];
             const Layer &F2 = net.layers[net.fc2];
-            e = value_head_f32(static_cast<const float *>(x), H.d_w32, H.d_b,
-                               F1.d_w32, F1.d_b, F2.d_w32, F2.d_b, raw, out, n,
-                               H.cin, S, F1.cout, s);
+            e = value_headconv_f32(static_cast<const float *>(x), H.d_w32,
+                                   H.d_b, ws.h0, n, H.cin, S, s);
+            if (e == hipSuccess) e = value_fc_head(ws.h0, F1.d_w32, F1.d_b,
+                                                   F2.d_w32, F2.d_b, raw, out,
+                                                   n, S * S, F1.cout, s);
         }
         if (e == hipSuccess && timer) e = timer->mark(s);
     }
@@ -297,8 +301,10 @@ int run_net(const Net &net, int which, int precision, int n,
             return LZ_ERR_UNSUPPORTED;
         }
         hipError_t e = timer ? timer->mark(stream) : hipSuccess;
+        FusedArgs args = net.fused;
+        args.h0 = ws.h0;
         if (e == hipSuccess)
-            e = fused_forward_bf16(which, d_planes, net.fused, d_raw, d_out, n,
+            e = fused_forward_bf16(which, d_planes, args, d_raw, d_out, n,
                                    temperature, net.fused_grid, stream);
         if (e == hipSuccess && timer) e = timer->mark(stream);
         if (e != hipSuccess) {

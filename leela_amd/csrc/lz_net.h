@@ -49,6 +49,7 @@ struct Workspace {
     float *act[2] = {nullptr, nullptr};
     float *raw = nullptr;
     float *out = nullptr;
+    float *h0 = nullptr;     // value head plane [cap][361]
 };
 
 int validate_net(Net &net, int which, std::string *msg);

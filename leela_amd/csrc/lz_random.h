@@ -14,9 +14,11 @@ class Rng {
 public:
     explicit Rng(uint32_t seed) { seedrandom(seed); }
 
-    // Random::seedrandom (Random.cpp:51-57).
+    // Random::seedrandom (Random.cpp:51-57). The first product is formed in
+    // 32-bit unsigned arithmetic (int literal times uint32) and then
+    // widened; the second multiplies the 64-bit state word.
     void seedrandom(uint32_t seed) {
-        s_[0] = 741103597ull * (uint64_t)seed;
+        s_[0] = (uint64_t)(uint32_t)(741103597u * seed);
         s_[1] = 741103597ull * s_[0];
     }
 

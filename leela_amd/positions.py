@@ -1,21 +1,141 @@
-"""Input batches for the benchmark and tests.
+"""Positions and input planes through the library's host-side C ABI.
 
-`synthetic_planes` builds plausible encoder-shaped plane stacks without a
-board engine."""
+Mirrors the reference's FastState / Network feature interface:
+``Position.play`` is FastState::play_move, ``Position.planes`` is
+Network::gather_features_policy / gather_features_value followed by the
+symmetry rotation of get_scored_moves_internal (Network.cpp:765-773).
+"""
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 
+from .backend import LZ_OK, LeelaError, load_library
+
+NET_POLICY, NET_VALUE = 0, 1
+PASS = -1
+QUERY_FIELDS = 12
+BLACK, WHITE, EMPTY = 0, 1, 2
+
+# Generator move caps: 0.6 * board area for the ragged mix, 250 / 200 for
+# the 19x19 batches.
+MAX_MOVES_19 = 250
+
+
+def _lib():
+    L = load_library()
+    if not getattr(L, "_positions_bound", False):
+        vp = C.c_void_p
+        u8p = C.POINTER(C.c_uint8)
+        i32p = C.POINTER(C.c_int32)
+        L.lz_position_new.restype = vp
+        L.lz_position_new.argtypes = [C.c_int, C.c_float]
+        L.lz_position_free.argtypes = [vp]
+        L.lz_position_free.restype = None
+        L.lz_position_play.argtypes = [vp, C.c_int]
+        L.lz_position_random.argtypes = [vp, C.c_uint32, C.c_int]
+        L.lz_position_info.argtypes = [vp, i32p]
+        L.lz_position_squares.argtypes = [vp, u8p]
+        L.lz_encode_planes.argtypes = [vp, C.c_int, C.c_int, u8p]
+        L.lz_board_queries.argtypes = [vp, i32p]
+        L.lz_make_batch.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                    C.c_int, C.c_int, u8p]
+        L._positions_bound = True
+    return L
+
+
+def _chk(rc, what):
+    if rc != LZ_OK:
+        raise LeelaError(rc, what)
+
+
+class Position:
+    """A game position (FastState counterpart)."""
+
+    def __init__(self, side: int = 19, komi: float = 7.5):
+        self._L = _lib()
+        self._p = self._L.lz_position_new(side, komi)
+        if not self._p:
+            raise ValueError(f"bad board side {side}")
+        self.side = side
+
+    def __del__(self):
+        try:
+            if self._p:
+                self._L.lz_position_free(self._p)
+        except Exception:
+            pass
+
+    @classmethod
+    def random(cls, seed: int, side: int = 19, max_moves: int = MAX_MOVES_19):
+        p = cls(side)
+        _chk(p._L.lz_position_random(p._p, seed, max_moves), "random")
+        return p
+
+    def vertex(self, x: int, y: int) -> int:
+        return (y + 1) * (self.side + 2) + (x + 1)
+
+    def play(self, vertex: int):
+        _chk(self._L.lz_position_play(self._p, vertex), f"play {vertex}")
+
+    def info(self) -> dict:
+        a = np.zeros(7, np.int32)
+        _chk(self._L.lz_position_info(self._p, a.ctypes.data_as(
+            C.POINTER(C.c_int32))), "info")
+        keys = ("side", "to_move", "komove", "last_move", "prev_last",
+                "movenum", "passes")
+        return dict(zip(keys, map(int, a)))
+
+    def squares(self) -> np.ndarray:
+        a = np.zeros(self.side * self.side, np.uint8)
+        _chk(self._L.lz_position_squares(self._p, a.ctypes.data_as(
+            C.POINTER(C.c_uint8))), "squares")
+        return a
+
+    def planes(self, net: int = NET_POLICY, symmetry: int = 0) -> np.ndarray:
+        a = np.zeros((32, self.side * self.side), np.uint8)
+        _chk(self._L.lz_encode_planes(self._p, net, symmetry,
+                                      a.ctypes.data_as(C.POINTER(C.c_uint8))),
+             "encode")
+        return a
+
+    def queries(self) -> np.ndarray:
+        a = np.zeros((self.side * self.side, QUERY_FIELDS), np.int32)
+        _chk(self._L.lz_board_queries(self._p, a.ctypes.data_as(
+            C.POINTER(C.c_int32))), "queries")
+        return a
+
+
+def make_batch(n: int, seed0: int = 1, side: int = 19,
+               max_moves: int = MAX_MOVES_19, net: int = NET_POLICY,
+               threads: int = 0) -> np.ndarray:
+    """Planes [n][32][side*side] of positions from seeds seed0..seed0+n-1."""
+    out = np.zeros((n, 32, side * side), np.uint8)
+    _chk(_lib().lz_make_batch(n, side, seed0, max_moves, net, threads,
+                              out.ctypes.data_as(C.POINTER(C.c_uint8))),
+         "make_batch")
+    return out
+
+
+def batch_planes(n: int, net: int = 0, seed0: int = 1) -> np.ndarray:
+    """Planes of the benchmark batch: random legal play from seeds
+    seed0..seed0+n-1, move counts uniform in 0..250, 19x19."""
+    return make_batch(n, seed0=seed0, side=19, max_moves=MAX_MOVES_19,
+                      net=net)
+
 
 def synthetic_planes(n: int, seed: int, side: int = 19) -> np.ndarray:
-    """uint8 [n][32][side*side] stacks shaped like the encoder's output:
-    planes 0/1/2 partition the board into empty / own / opponent points, the
-    liberty planes mark stones, the after-move planes mark empties, and the
-    remaining planes are sparse. Used where the values only need to be
-    plausible inputs (kernel parity), not positions of a real game."""
+    """uint8 [n][32][side*side] stacks shaped like the encoder's output but
+    drawn independently per plane (no board engine): planes 0/1/2 partition
+    the board, liberty planes mark stones, after-move planes mark empties,
+    the rest are sparse. For kernel tests that need arbitrary inputs."""
     rng = np.random.default_rng(seed)
     P = side * side
     out = np.zeros((n, 32, P), np.uint8)
+    xs = np.arange(P) % side
+    ys = np.arange(P) // side
+    line3 = (xs == 2) | (xs == side - 3) | (ys == 2) | (ys == side - 3)
     for i in range(n):
         fill = rng.uniform(0.0, 0.6)
         occ = rng.random(P)
@@ -34,12 +154,5 @@ def synthetic_planes(n: int, seed: int, side: int = 19) -> np.ndarray:
             out[i, 19 + k] = emp & (at == k)
         out[i, 25:30] = (rng.random((5, P)) < 0.01) & emp
         out[i, 30] = opp
-        xs = np.arange(P) % side
-        ys = np.arange(P) // side
-        out[i, 31] = (xs == 2) | (xs == side - 3) | (ys == 2) | (ys == side - 3)
+        out[i, 31] = line3
     return out
-
-
-def batch_planes(n: int, net: int = 0, seed0: int = 1) -> np.ndarray:
-    """Planes of the benchmark batch."""
-    return synthetic_planes(n, seed0)

@@ -30,10 +30,11 @@ static inline uint64_t rotl64(uint64_t x, int k) {
     return (x << k) | (x >> (64 - k));
 }
 
-/* Random::seedrandom (Random.cpp:51-57): 32-bit LCG multiplier, products
- * taken in 64-bit unsigned arithmetic as the reference's uint64 members. */
+/* Random::seedrandom (Random.cpp:51-57): `741103597 * seed` multiplies an
+ * int literal by a uint32, so it wraps at 32 bits before it is stored in
+ * the uint64 state word; the second product is 64-bit. */
 void orc_rng_seed(uint64_t st[2], uint32_t seed) {
-    st[0] = (uint64_t)741103597u * (uint64_t)seed;
+    st[0] = (uint64_t)(uint32_t)(741103597u * seed);
     st[1] = (uint64_t)741103597u * st[0];
 }
 
