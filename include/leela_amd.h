@@ -41,6 +41,8 @@ extern "C" {
 #define LZ_PREC_FP32 0 /* plain form: FP32 everywhere, FP32 MFMA */
 #define LZ_PREC_BF16 1 /* BF16 trunk on MFMA, FP32 accumulate/epilogue */
 #define LZ_PREC_FP8 2  /* FP8 e4m3 trunk, per-output-channel weight scales */
+#define LZ_PREC_FP16 3 /* FP16 operands on the BF16 kernels' MFMA path:
+                          same rate, 3 more mantissa bits */
 
 typedef struct lz_ctx lz_ctx;
 

@@ -25,8 +25,9 @@ LZ_ERR_NOMEM = -4
 LZ_ERR_UNSUPPORTED = -5
 
 NET_POLICY, NET_VALUE = 0, 1
-PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
-PRECISIONS = {"fp32": PREC_FP32, "bf16": PREC_BF16, "fp8": PREC_FP8}
+PREC_FP32, PREC_BF16, PREC_FP8, PREC_FP16 = 0, 1, 2, 3
+PRECISIONS = {"fp32": PREC_FP32, "bf16": PREC_BF16, "fp8": PREC_FP8,
+              "fp16": PREC_FP16}
 
 
 class LeelaError(RuntimeError):

@@ -238,8 +238,9 @@ __global__ __launch_bounds__(kThreads) void value_headconv_f32_kernel(
 // Batched inner-product head of the value net (Network.cpp:733-736):
 // hidden = ELU(W1 h0 + b1), v = w2 . hidden + b2, out = (1 + tanh v) / 2.
 // A workgroup takes kFcBoards boards so every W1 element loaded from L2 is
-// used kFcBoards times; thread o accumulates output o for all of them.
-constexpr int kFcBoards = 16;
+// used kFcBoards times; thread o accumulates output o for all of them. Four
+// boards per workgroup keeps a batch of 1024 on 256 workgroups (one per CU).
+constexpr int kFcBoards = 4;
 
 __global__ __launch_bounds__(kThreads) void value_fc_kernel(
     const float *__restrict__ h0, const float *__restrict__ w1t,
@@ -262,6 +263,7 @@ __global__ __launch_bounds__(kThreads) void value_fc_kernel(
         float acc[kFcBoards];
 #pragma unroll
         for (int bb = 0; bb < kFcBoards; bb++) acc[bb] = 0.0f;
+#pragma unroll 8
         for (int i = 0; i < P; i++) {
             const float wv = w1t[(size_t)i * H + o];
 #pragma unroll

@@ -1,41 +1,58 @@
-// fused_bf16.hip — persistent fused layer-chain kernel: all convs of the
-// policy (or value) net for one board per workgroup pass, BF16 on MFMA with
-// FP32 accumulation, activations resident in LDS.
+// fused_net.hip — persistent fused layer-chain kernel: all convs of the
+// policy (or value) net for one board per workgroup pass, 16-bit operands
+// (BF16, or FP16) on MFMA with FP32 accumulation, activations resident in
+// LDS.
 //
 // Work decomposition
 //   * A workgroup owns one board at a time and loops over boards
-//     (persistent grid). The board's activations live in LDS as BF16 in an
-//     HWC tile with a zero halo: [SP][SP][PS], SP = 19 + 2*pad, PS = C + 8
-//     (the 8-element pad staggers pixels across LDS banks).
+//     (persistent grid). The board's activations live in LDS as 16-bit
+//     values in an HWC tile with a zero halo: [SP][SP][PS], SP = 19 + 2*pad,
+//     PS = C + 8 (the 8-element pad staggers pixels across LDS banks).
 //   * Each conv is an implicit GEMM D[cout x 361] = W[cout x K] B[K x 361],
 //     K = k*k*cin ordered tap-major, 32 channels per MFMA k-step
-//     (v_mfma_f32_16x16x32_bf16). The waves form a WM (output channels) x 4
-//     (pixels) grid (policy 2 x 4, value 1 x 4); a wave owns 1/WM of the
-//     output channels of six 16-pixel column tiles (4 x 6 x 16 = 384 >= 361
-//     pixels). Its B fragments are
-//     one 16-byte ds_read per lane at pixel_base + tap_offset +
-//     channel_block, prefetched one k-step ahead.
+//     (v_mfma_f32_16x16x32_{bf16,f16}). The waves form a WM (output
+//     channels) x 4 (pixels) grid (policy 2 x 4, value 1 x 4); a wave owns
+//     1/WM of the output channels of six 16-pixel column tiles
+//     (4 x 6 x 16 = 384 >= 361 pixels). Its B fragments are one 16-byte
+//     ds_read per lane at pixel_base + tap_offset + channel_block,
+//     prefetched one k-step ahead.
 //   * Weights are pre-packed on the host in MFMA fragment order, [k-step]
 //     [m-tile][lane][8], so an A operand is one coalesced 16-byte global load
-//     per lane, prefetched one k-step ahead. The four pixel-waves of a
-//     channel half walk the same k-steps together, so repeated fragment
-//     reads hit in L1 and each CU pulls a layer's weights from L2 once per
-//     board.
-//   * Epilogue: bias + ELU in FP32 (Network.cpp:382-392), rounded to BF16,
-//     written back over the input tile after a barrier. The last 1-channel
-//     conv keeps FP32: softmax(T) over the logits (Network.cpp:450-469) for
-//     the policy net; for the value net the ELU'd plane goes to HBM and the
-//     batched inner-product head (value_fc_head) finishes it.
+//     per lane, prefetched one k-step ahead. The pixel-waves of a channel
+//     group walk the same k-steps together, so repeated fragment reads hit
+//     in L1 and each CU pulls a layer's weights from L2 once per board.
+//   * Epilogue: bias + ELU in FP32 (Network.cpp:382-392), rounded to the
+//     operand type, written back over the input tile after a barrier. The
+//     last 1-channel conv keeps FP32: softmax(T) over the logits
+//     (Network.cpp:450-469) for the policy net; for the value net the ELU'd
+//     plane goes to HBM and the batched inner-product head (value_fc_head)
+//     finishes it.
 #include "lz_fused.h"
 #include "lz_internal.h"
 
 namespace lz {
 namespace {
 
-typedef __bf16 bf16;
-typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Op;
+template <>
+struct Op<__bf16> {
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    typedef __bf16 v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ f32x4 mfma(v8 a, v8 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Op<_Float16> {
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ f32x4 mfma(v8 a, v8 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
 
 constexpr int kWN = 4;                // waves along pixels
 constexpr int kNT = 6;                // 16-pixel tiles per wave
@@ -46,27 +63,24 @@ __device__ __forceinline__ float elu(float v) {
     return v > 0.0f ? v : (expf(v) - 1.0f);
 }
 
-// Offset (in bf16 elements) of pixel j's window origin in a tile of side SP
-// and pixel stride PS. Padding columns (j >= 361) alias pixel 0.
+// Offset (in elements) of pixel j's window origin in a tile of side SP and
+// pixel stride PS. Padding columns (j >= 361) alias pixel 0.
 template <int SP, int PS>
 __device__ __forceinline__ int pix_base(int j) {
     if (j >= kP) j = 0;
     return ((j / 19) * SP + (j % 19)) * PS;
 }
 
-__device__ __forceinline__ const bf16x8 *wfrag(const FusedArgs &a, int l) {
-    return static_cast<const bf16x8 *>(a.w[l]);
-}
-
 // One same-padded conv of the tile `act` (side SP, stride PS, CIN input
 // channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the packed
 // weights) for the wave's column tiles.
-template <int F, int CIN, int MT, int MTW, int SP, int PS>
-__device__ __forceinline__ void conv_tile(const bf16 *act,
-                                          const bf16x8 *__restrict__ wpk,
+template <typename T, int F, int CIN, int MT, int MTW, int SP, int PS>
+__device__ __forceinline__ void conv_tile(const T *act,
+                                          const typename Op<T>::v8 *__restrict__ wpk,
                                           f32x4 (&acc)[MTW][kNT],
                                           const int (&pb)[kNT], int mt0,
                                           int lane) {
+    typedef typename Op<T>::v8 v8;
     constexpr int CB = CIN / 32;
     constexpr int STEPS = F * F * CB;
     const int g8 = (lane >> 4) * 8;
@@ -75,28 +89,27 @@ __device__ __forceinline__ void conv_tile(const bf16 *act,
 #pragma unroll
         for (int t = 0; t < kNT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const bf16x8 *wl = wpk + mt0 * 64 + lane;
-    bf16x8 a[MTW], b[kNT];
+    const v8 *wl = wpk + mt0 * 64 + lane;
+    v8 a[MTW], b[kNT];
 #pragma unroll
     for (int m = 0; m < MTW; m++) a[m] = wl[m * 64];
 #pragma unroll
     for (int t = 0; t < kNT; t++)
-        b[t] = *reinterpret_cast<const bf16x8 *>(act + pb[t] + g8);
+        b[t] = *reinterpret_cast<const v8 *>(act + pb[t] + g8);
 
     for (int s = 0; s < STEPS; s++) {
         const int nxt = s + 1 < STEPS ? s + 1 : s;
         const int tap = nxt / CB, cb = nxt % CB;
         const int off = ((tap / F) * SP + (tap % F)) * PS + cb * 32 + g8;
-        bf16x8 bn[kNT];
+        v8 bn[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++)
-            bn[t] = *reinterpret_cast<const bf16x8 *>(act + pb[t] + off);
+            bn[t] = *reinterpret_cast<const v8 *>(act + pb[t] + off);
 #pragma unroll
         for (int m = 0; m < MTW; m++) {
 #pragma unroll
             for (int t = 0; t < kNT; t++)
-                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    a[m], b[t], acc[m][t], 0, 0, 0);
+                acc[m][t] = Op<T>::mfma(a[m], b[t], acc[m][t]);
             a[m] = wl[(nxt * MT + m) * 64];
         }
 #pragma unroll
@@ -104,15 +117,16 @@ __device__ __forceinline__ void conv_tile(const bf16 *act,
     }
 }
 
-// Bias + ELU of acc, written as BF16 into the trunk tile (side 21, stride
-// PSO). D layout of 16x16 MFMA: lane holds rows 4*(lane/16)+r of column
-// lane%16, i.e. four consecutive output channels of one pixel.
-template <int MTW, int PSO>
-__device__ __forceinline__ void store_tile(bf16 *act,
+// Bias + ELU of acc, written into the trunk tile (side 21, stride PSO).
+// D layout of 16x16 MFMA: lane holds rows 4*(lane/16)+r of column lane%16,
+// i.e. four consecutive output channels of one pixel.
+template <typename T, int MTW, int PSO>
+__device__ __forceinline__ void store_tile(T *act,
                                            const f32x4 (&acc)[MTW][kNT],
                                            const float *__restrict__ bias,
                                            int cout, int mt0, int nt0,
                                            int lane) {
+    typedef typename Op<T>::v4 v4;
     const int g4 = (lane >> 4) * 4;
 #pragma unroll
     for (int m = 0; m < MTW; m++) {
@@ -124,11 +138,11 @@ __device__ __forceinline__ void store_tile(bf16 *act,
         for (int t = 0; t < kNT; t++) {
             const int j = (nt0 + t) * 16 + (lane & 15);
             if (j >= kP) continue;
-            bf16x4 v;
+            v4 v;
 #pragma unroll
-            for (int r = 0; r < 4; r++) v[r] = (bf16)elu(acc[m][t][r] + bv[r]);
+            for (int r = 0; r < 4; r++) v[r] = (T)elu(acc[m][t][r] + bv[r]);
             const int y = j / 19 + 1, x = j % 19 + 1;
-            *reinterpret_cast<bf16x4 *>(act + (y * 21 + x) * PSO + o0) = v;
+            *reinterpret_cast<v4 *>(act + (y * 21 + x) * PSO + o0) = v;
         }
     }
 }
@@ -155,20 +169,21 @@ __device__ float block_max(float v, float *s_red) {
     return r;
 }
 
-// C: trunk width (128 policy, 64 value); M1: outputs of the 5x5 stem;
-// NMID: 3x3 convs between stem and head (the first takes M1 inputs).
-// Policy: writes raw = ELU'd logits and out = softmax(raw / T).
+// T: operand type; C: trunk width (128 policy, 64 value); M1: outputs of
+// the 5x5 stem; NMID: 3x3 convs between stem and head (the first takes M1
+// inputs). Policy: writes raw = ELU'd logits and out = softmax(raw / T).
 // Value: writes the ELU'd head plane to h0 [n][361] for value_fc_head.
 // kWM waves split the output channels, kWN = 4 the pixels; OCC is the
 // waves per SIMD the register budget is sized for.
-template <int C, int M1, int NMID, bool VALUE, int kWM, int OCC>
-__global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net_bf16(
+template <typename T, int C, int M1, int NMID, bool VALUE, int kWM, int OCC>
+__global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
     const uint8_t *__restrict__ planes, FusedArgs args,
     float *__restrict__ raw, float *__restrict__ out, int n,
     float temperature) {
+    typedef typename Op<T>::v8 v8;
     constexpr int PS = C + 8;
     constexpr int SP = 21;
-    constexpr int TILE = SP * SP * PS;         // trunk tile (bf16 elements)
+    constexpr int TILE = SP * SP * PS;         // trunk tile (elements)
     constexpr int TILE0 = 23 * 23 * kInPS;     // stem input tile
     constexpr int TILE_MAX = TILE > TILE0 ? TILE : TILE0;
     constexpr int kWaves = kWM * kWN;
@@ -178,13 +193,14 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net_bf16(
     static_assert(C % 32 == 0 && M1 % 32 == 0 && MT % kWM == 0 &&
                       MT1 % kWM == 0, "channel blocking");
 
-    __shared__ __attribute__((aligned(16))) bf16 s_act[TILE_MAX];
+    __shared__ __attribute__((aligned(16))) T s_act[TILE_MAX];
     __shared__ float s_vec[384];
     __shared__ float s_red[kWaves];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / kWN, wn = wave % kWN;
     const int nt0 = wn * kNT;
+    auto W = [&](int l) { return static_cast<const v8 *>(args.w[l]); };
 
     int pb_in[kNT], pb[kNT];
 #pragma unroll
@@ -195,57 +211,59 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net_bf16(
     }
 
     for (int board = blockIdx.x; board < n; board += gridDim.x) {
-        // Stage the 32 binary planes as BF16 with a 2-wide zero halo.
+        // Stage the 32 binary planes with a 2-wide zero halo.
         const uint8_t *src = planes + (size_t)board * 32 * kP;
         for (int i = tid; i < 23 * 23 * 4; i += kThreads) {
             const int pp = i >> 2, cg = (i & 3) * 8;
             const int y = pp / 23 - 2, x = pp % 23 - 2;
             const bool in = y >= 0 && y < 19 && x >= 0 && x < 19;
-            bf16x8 v;
+            v8 v;
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                v[e] = (bf16)(in ? (float)src[(cg + e) * kP + y * 19 + x] : 0.0f);
-            *reinterpret_cast<bf16x8 *>(s_act + pp * kInPS + cg) = v;
+                v[e] = (T)(in ? (float)src[(cg + e) * kP + y * 19 + x] : 0.0f);
+            *reinterpret_cast<v8 *>(s_act + pp * kInPS + cg) = v;
         }
         __syncthreads();
 
         // Stem: 5x5, 32 -> M1.
         {
             f32x4 acc[MTW1][kNT];
-            conv_tile<5, 32, MT1, MTW1, 23, kInPS>(s_act, wfrag(args, 0), acc,
-                                                   pb_in, wm * MTW1, lane);
+            conv_tile<T, 5, 32, MT1, MTW1, 23, kInPS>(s_act, W(0), acc, pb_in,
+                                                      wm * MTW1, lane);
             __syncthreads();
             // Switch to the trunk geometry: clear tile and halo, then write.
             for (int i = tid * 8; i < TILE; i += kThreads * 8)
-                *reinterpret_cast<bf16x8 *>(s_act + i) = bf16x8{};
+                *reinterpret_cast<v8 *>(s_act + i) = v8{};
             __syncthreads();
-            store_tile<MTW1, PS>(s_act, acc, args.b[0], M1, wm * MTW1, nt0,
-                                 lane);
+            store_tile<T, MTW1, PS>(s_act, acc, args.b[0], M1, wm * MTW1, nt0,
+                                    lane);
             __syncthreads();
         }
         // Middle 3x3 convs; the first reads M1 channels.
         {
             f32x4 acc[MTW][kNT];
-            conv_tile<3, M1, MT, MTW, SP, PS>(s_act, wfrag(args, 1), acc, pb,
-                                              wm * MTW, lane);
+            conv_tile<T, 3, M1, MT, MTW, SP, PS>(s_act, W(1), acc, pb,
+                                                 wm * MTW, lane);
             __syncthreads();
-            store_tile<MTW, PS>(s_act, acc, args.b[1], C, wm * MTW, nt0, lane);
+            store_tile<T, MTW, PS>(s_act, acc, args.b[1], C, wm * MTW, nt0,
+                                   lane);
             __syncthreads();
         }
         for (int l = 2; l <= NMID; l++) {
             f32x4 acc[MTW][kNT];
-            conv_tile<3, C, MT, MTW, SP, PS>(s_act, wfrag(args, l), acc, pb,
-                                             wm * MTW, lane);
+            conv_tile<T, 3, C, MT, MTW, SP, PS>(s_act, W(l), acc, pb,
+                                                wm * MTW, lane);
             __syncthreads();
-            store_tile<MTW, PS>(s_act, acc, args.b[l], C, wm * MTW, nt0, lane);
+            store_tile<T, MTW, PS>(s_act, acc, args.b[l], C, wm * MTW, nt0,
+                                   lane);
             __syncthreads();
         }
-        // Head: 3x3, C -> 1, in FP32; the channel-half-0 waves cover all
+        // Head: 3x3, C -> 1, in FP32; the channel-group-0 waves cover all
         // pixels.
         if (wm == 0) {
             f32x4 acc[1][kNT];
-            conv_tile<3, C, 1, 1, SP, PS>(s_act, wfrag(args, NMID + 1), acc,
-                                          pb, 0, lane);
+            conv_tile<T, 3, C, 1, 1, SP, PS>(s_act, W(NMID + 1), acc, pb, 0,
+                                             lane);
             const float hb = args.b[NMID + 1][0];
             if (lane < 16) {
 #pragma unroll
@@ -279,51 +297,64 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net_bf16(
     }
 }
 
-// Grid occupancy: the policy tile (121 KB LDS) allows one workgroup per
-// CU, the value tile (65 KB) two.
 // Policy: one 8-wave workgroup per CU (121 KB LDS tile), 2 waves per SIMD.
 // Value: 4-wave workgroups, two per CU (65 KB tile each), 2 waves per SIMD.
 constexpr int kPolicyWM = 2, kValueWM = 1;
 constexpr int kPolicyThreads = kPolicyWM * kWN * 64;
 constexpr int kValueThreads = kValueWM * kWN * 64;
 
-#define POLICY_KERNEL fused_net_bf16<128, 96, 11, false, kPolicyWM, 2>
-#define VALUE_KERNEL fused_net_bf16<64, 64, 10, true, kValueWM, 2>
+template <typename T>
+struct Kernels {
+    static constexpr auto policy = fused_net<T, 128, 96, 11, false, kPolicyWM, 2>;
+    static constexpr auto value = fused_net<T, 64, 64, 10, true, kValueWM, 2>;
+};
 
-}  // namespace
-
-int fused_blocks_per_cu(int which) {
-    int blocks = 0;
-    hipError_t e =
-        which == 0
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, POLICY_KERNEL, kPolicyThreads, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, VALUE_KERNEL, kValueThreads, 0);
-    return (e == hipSuccess && blocks > 0) ? blocks : 1;
-}
-
-hipError_t fused_forward_bf16(int which, const uint8_t *planes,
-                              const FusedArgs &args, float *raw, float *out,
-                              int n, float temperature, int grid,
-                              hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    if (grid <= 0 || grid > n) grid = n;
+template <typename T>
+hipError_t launch(int which, const uint8_t *planes, const FusedArgs &args,
+                  float *raw, float *out, int n, float temperature, int grid,
+                  hipStream_t stream) {
     if (which == 0) {
         if (args.nconv != 13 || args.c != 128 || args.m1 != 96)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL(POLICY_KERNEL, dim3(grid), dim3(kPolicyThreads), 0,
-                           stream, planes, args, raw, out, n, temperature);
+        hipLaunchKernelGGL(Kernels<T>::policy, dim3(grid),
+                           dim3(kPolicyThreads), 0, stream, planes, args, raw,
+                           out, n, temperature);
         return hipGetLastError();
     }
     if (args.nconv != 12 || args.c != 64 || args.m1 != 64 || !args.h0)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(VALUE_KERNEL, dim3(grid), dim3(kValueThreads), 0, stream,
-                       planes, args, raw, out, n, temperature);
+    hipLaunchKernelGGL(Kernels<T>::value, dim3(grid), dim3(kValueThreads), 0,
+                       stream, planes, args, raw, out, n, temperature);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return value_fc_head(args.h0, args.fc1_wt, args.fc1_b, args.fc2_w,
                          args.fc2_b, raw, out, n, kP, args.hidden, stream);
+}
+
+}  // namespace
+
+int fused_blocks_per_cu(int which) {
+    // The BF16 and FP16 instantiations share shapes and resources.
+    int blocks = 0;
+    hipError_t e =
+        which == 0
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &blocks, Kernels<__bf16>::policy, kPolicyThreads, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &blocks, Kernels<__bf16>::value, kValueThreads, 0);
+    return (e == hipSuccess && blocks > 0) ? blocks : 1;
+}
+
+hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
+                         const FusedArgs &args, float *raw, float *out, int n,
+                         float temperature, int grid, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    if (grid <= 0 || grid > n) grid = n;
+    if (type == FusedType::kBF16)
+        return launch<__bf16>(which, planes, args, raw, out, n, temperature,
+                              grid, stream);
+    return launch<_Float16>(which, planes, args, raw, out, n, temperature,
+                            grid, stream);
 }
 
 }  // namespace lz

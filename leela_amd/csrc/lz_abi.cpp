@@ -182,8 +182,7 @@ static int forward_locked(lz_ctx *ctx, int net, int precision, int n,
         return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
     if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
         return ctx->fail(LZ_ERR_ARG, "bad net id %d", net);
-    if (precision != LZ_PREC_FP32 && precision != LZ_PREC_BF16 &&
-        precision != LZ_PREC_FP8)
+    if (precision < LZ_PREC_FP32 || precision > LZ_PREC_FP16)
         return ctx->fail(LZ_ERR_ARG, "bad precision %d", precision);
     if (n < 0) return ctx->fail(LZ_ERR_ARG, "negative batch");
     if (n == 0) return LZ_OK;

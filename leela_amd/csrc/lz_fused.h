@@ -9,7 +9,7 @@ namespace lz {
 constexpr int kMaxFusedConvs = 16;
 
 // Device pointers of one net in the fused kernels' layouts. w[l] holds conv
-// l packed in MFMA fragment order: [k-step][m-tile][lane 0..63][8 values],
+// l packed (in the operand type) in MFMA fragment order: [k-step][m-tile][lane 0..63][8 values],
 // k-steps tap-major with 32 input channels each (see pack_conv_fragments).
 struct FusedArgs {
     const void *w[kMaxFusedConvs];
@@ -25,6 +25,10 @@ struct FusedArgs {
     int m1 = 0;  // outputs of the 5x5 stem
 };
 
+// MFMA operand type of the fused kernels: the conv weights are packed in
+// it and the LDS activation tile holds it.
+enum class FusedType { kBF16 = 0, kFP16 = 1 };
+
 // which: 0 policy (13 convs: 5x5 32->96, 3x3 96->128, 10x 128->128,
 // 128->1), 1 value (12 convs: 5x5 32->64, 10x 64->64, 64->1, + 2 FC).
 // grid <= 0 launches one workgroup per board.
@@ -32,9 +36,8 @@ struct FusedArgs {
 // register limited); the persistent grid is CUs times this.
 int fused_blocks_per_cu(int which);
 
-hipError_t fused_forward_bf16(int which, const uint8_t *planes,
-                              const FusedArgs &args, float *raw, float *out,
-                              int n, float temperature, int grid,
-                              hipStream_t stream);
+hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
+                         const FusedArgs &args, float *raw, float *out, int n,
+                         float temperature, int grid, hipStream_t stream);
 
 }  // namespace lz

@@ -136,12 +136,16 @@ int upload_net(Net &net, int which, std::string *msg) {
     if (!ok) return LZ_OK;
     FusedArgs &A = net.fused;
     for (int k = 0; k < nconv && e == hipSuccess; k++) {
-        e = upload(&net.layers[k].d_wbf16, pack_conv_fragments(L[k]));
+        e = upload(&net.layers[k].d_wbf16,
+                   pack_conv_fragments(L[k], FusedType::kBF16));
+        if (e == hipSuccess)
+            e = upload(&net.layers[k].d_wf16,
+                       pack_conv_fragments(L[k], FusedType::kFP16));
         A.w[k] = net.layers[k].d_wbf16;
         A.b[k] = net.layers[k].d_b;
     }
     if (e != hipSuccess) {
-        *msg = std::string("BF16 weight upload: ") + hipGetErrorString(e);
+        *msg = std::string("16-bit weight upload: ") + hipGetErrorString(e);
         return LZ_ERR_HIP;
     }
     A.nconv = nconv;
@@ -154,6 +158,8 @@ int upload_net(Net &net, int which, std::string *msg) {
         A.fc2_b = L[net.fc2].d_b;
         A.hidden = L[net.fc1].cout;
     }
+    net.fused16 = A;
+    for (int k = 0; k < nconv; k++) net.fused16.w[k] = net.layers[k].d_wf16;
     int dev = 0, cus = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -170,7 +176,16 @@ uint16_t float_to_bf16_rne(float f) {
     return (uint16_t)(u >> 16);
 }
 
-std::vector<uint16_t> pack_conv_fragments(const Layer &L) {
+uint16_t float_to_f16_rne(float f) {
+    // The host compiler's _Float16 conversion rounds to nearest even and
+    // handles subnormals, overflow and NaN.
+    const _Float16 h = (_Float16)f;
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+std::vector<uint16_t> pack_conv_fragments(const Layer &L, FusedType type) {
     const int F = L.filter, cin = L.cin, cout = L.cout;
     const int CB = cin / 32, steps = F * F * CB, MT = (cout + 15) / 16;
     std::vector<uint16_t> out((size_t)steps * MT * 64 * 8, 0);
@@ -182,9 +197,13 @@ std::vector<uint16_t> pack_conv_fragments(const Layer &L) {
                 for (int e = 0; e < 8; e++, idx++) {
                     const int o = m * 16 + (lane & 15);
                     const int ch = cb * 32 + (lane >> 4) * 8 + e;
-                    if (o < cout)
-                        out[idx] = float_to_bf16_rne(
-                            L.w[(((size_t)o * cin + ch) * F + dy) * F + dx]);
+                    if (o < cout) {
+                        const float w =
+                            L.w[(((size_t)o * cin + ch) * F + dy) * F + dx];
+                        out[idx] = type == FusedType::kBF16
+                                       ? float_to_bf16_rne(w)
+                                       : float_to_f16_rne(w);
+                    }
                 }
     }
     return out;
@@ -195,8 +214,9 @@ void free_net(Net &net) {
         dfree(L.d_w32);
         dfree(L.d_b);
         dfree(L.d_wbf16);
+        dfree(L.d_wf16);
         L.d_w32 = L.d_b = nullptr;
-        L.d_wbf16 = nullptr;
+        L.d_wbf16 = L.d_wf16 = nullptr;
     }
 }
 
@@ -295,20 +315,24 @@ int run_net(const Net &net, int which, int precision, int n,
     if (precision == LZ_PREC_FP32)
         return run_fp32(net, which, n, d_planes, d_raw, d_out, temperature,
                         ws, stream, msg, timer);
-    if (precision == LZ_PREC_BF16) {
+    if (precision == LZ_PREC_BF16 || precision == LZ_PREC_FP16) {
         if (!net.fused_ok) {
-            *msg = "BF16 path implements only the reference layer shapes";
+            *msg = "16-bit fused path implements only the reference layer "
+                   "shapes";
             return LZ_ERR_UNSUPPORTED;
         }
+        const bool bf = precision == LZ_PREC_BF16;
         hipError_t e = timer ? timer->mark(stream) : hipSuccess;
-        FusedArgs args = net.fused;
+        FusedArgs args = bf ? net.fused : net.fused16;
         args.h0 = ws.h0;
         if (e == hipSuccess)
-            e = fused_forward_bf16(which, d_planes, args, d_raw, d_out, n,
-                                   temperature, net.fused_grid, stream);
+            e = fused_forward(which, bf ? FusedType::kBF16 : FusedType::kFP16,
+                              d_planes, args, d_raw, d_out, n, temperature,
+                              net.fused_grid, stream);
         if (e == hipSuccess && timer) e = timer->mark(stream);
         if (e != hipSuccess) {
-            *msg = std::string("BF16 forward launch: ") + hipGetErrorString(e);
+            *msg = std::string("16-bit forward launch: ") +
+                   hipGetErrorString(e);
             return LZ_ERR_HIP;
         }
         return LZ_OK;

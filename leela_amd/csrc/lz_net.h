@@ -25,6 +25,7 @@ struct Layer {
                              // OIHW; FC1 transposed [in][out]; FC2 as is
     float *d_b = nullptr;
     uint16_t *d_wbf16 = nullptr;  // BF16 MFMA fragments (fused kernels)
+    uint16_t *d_wf16 = nullptr;   // FP16 MFMA fragments (fused kernels)
 };
 
 // A net as the reference pushes it (Network.cpp:206-233): a chain of convs
@@ -40,6 +41,7 @@ struct Net {
     int max_ch = 0;     // widest trunk activation
     bool fused_ok = false;  // shapes the fused kernels implement
     FusedArgs fused;        // device pointers for the BF16 fused kernel
+    FusedArgs fused16;      // device pointers for the FP16 fused kernel
     int fused_grid = 0;     // persistent workgroups (CUs x occupancy)
 };
 
@@ -71,11 +73,13 @@ int run_net(const Net &net, int which, int precision, int n,
             float temperature, Workspace &ws, hipStream_t stream,
             std::string *msg, LaunchTimer *timer = nullptr);
 
-// Host packing of a conv into MFMA fragment order (BF16 bits, RNE):
-// [k-step][m-tile][lane][8], k-step s = tap * (cin/32) + channel block,
-// lane = row + 16 * g holding input channels 32*cb + 8*g .. +7.
-std::vector<uint16_t> pack_conv_fragments(const Layer &L);
+// Host packing of a conv into MFMA fragment order (16-bit values, round to
+// nearest even): [k-step][m-tile][lane][8], k-step s = tap * (cin/32) +
+// channel block, lane = row + 16 * g holding input channels
+// 32*cb + 8*g .. +7.
+std::vector<uint16_t> pack_conv_fragments(const Layer &L, FusedType type);
 uint16_t float_to_bf16_rne(float f);
+uint16_t float_to_f16_rne(float f);
 
 void normal_fill(uint32_t seed, size_t n, float stddev, float *out);
 

@@ -1,0 +1,111 @@
+"""GPU checks of the fused 16-bit layer-chain kernels (BF16 and FP16
+operands, FP32 accumulation, FP32 bias/ELU/softmax/tanh) against the FP64
+oracle (direct convolution) on positions from the generator.
+
+Tolerances, 64 positions of 19x19 (seeds 301..364, up to 250 moves):
+  BF16: top-1 agreement >= 95%, max |d prob| < 0.02, mean KL < 1e-2,
+        max |d value| < 0.02 (8-bit mantissa; He-normal nets have small
+        top-2 gaps, so BF16 flips some near-ties);
+  FP16: top-1 agreement >= 99%, max |d prob| < 0.005, mean KL < 1e-3,
+        max |d value| < 0.005.
+"""
+import numpy as np
+import pytest
+
+from leela_amd.positions import make_batch, synthetic_planes
+
+pytestmark = pytest.mark.gpu
+
+BARS = {  # precision: (top1, dprob, kl, dvalue)
+    "bf16": (0.95, 0.02, 1e-2, 0.02),
+    "fp16": (0.99, 0.005, 1e-3, 0.005),
+}
+
+
+@pytest.fixture(scope="module")
+def gpu(weights_b):
+    import leela_amd
+    be = leela_amd.Backend(0)
+    pol, val = weights_b
+    be.load_net(leela_amd.NET_POLICY, pol)
+    be.load_net(leela_amd.NET_VALUE, val)
+    be.finalize()
+    yield be
+    be.close()
+
+
+@pytest.fixture(scope="module")
+def planes64():
+    return make_batch(64, seed0=301)
+
+
+@pytest.fixture(scope="module")
+def ref64(weights_b, planes64):
+    import oracle
+    pol, val = weights_b
+    p = oracle.forward_f64(oracle.Net(pol), planes64, oracle.HEAD_POLICY)
+    v = oracle.forward_f64(oracle.Net(val), planes64, oracle.HEAD_VALUE)
+    return p, v
+
+
+def kl(p, q):
+    q = np.maximum(q, 1e-30)
+    p = np.maximum(p, 1e-30)
+    return np.sum(p * (np.log(p) - np.log(q)), axis=1)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_policy_vs_fp64(gpu, planes64, ref64, prec):
+    import leela_amd
+    (raw64, p64), _ = ref64
+    raw, prob = gpu.forward(leela_amd.NET_POLICY, leela_amd.PRECISIONS[prec],
+                            planes64)
+    top1 = np.mean(raw.argmax(1) == raw64.argmax(1))
+    dprob = np.abs(prob - p64).max()
+    k = kl(p64, prob.astype(np.float64)).mean()
+    print(f"{prec} policy: top1={top1:.4f} max|dprob|={dprob:.3g} KL={k:.3g} "
+          f"max|dlogit|={np.abs(raw - raw64).max():.3g}")
+    b_top1, b_dprob, b_kl, _ = BARS[prec]
+    assert top1 >= b_top1
+    assert dprob < b_dprob
+    assert k < b_kl
+    np.testing.assert_allclose(prob.sum(1), 1.0, atol=1e-5)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_value_vs_fp64(gpu, planes64, ref64, prec):
+    import leela_amd
+    _, (v64, w64) = ref64
+    v, w = gpu.forward(leela_amd.NET_VALUE, leela_amd.PRECISIONS[prec],
+                       planes64)
+    err = np.abs(w - w64).max()
+    print(f"{prec} value: max|dwin|={err:.3g} max|draw|={np.abs(v - v64).max():.3g}")
+    assert err < BARS[prec][3]
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_rows_independent_of_batch(gpu, prec):
+    import leela_amd
+    p = leela_amd.PRECISIONS[prec]
+    planes = make_batch(40, seed0=9)
+    a, _ = gpu.forward(leela_amd.NET_POLICY, p, planes)
+    b, _ = gpu.forward(leela_amd.NET_POLICY, p, planes[[39, 0, 17]])
+    assert np.array_equal(a[[39, 0, 17]], b)
+    va, _ = gpu.forward(leela_amd.NET_VALUE, p, planes)
+    vb, _ = gpu.forward(leela_amd.NET_VALUE, p, planes[[5]])
+    assert np.array_equal(va[[5]], vb)
+
+
+def test_large_batch_persistent_grid(gpu):
+    """More boards than resident workgroups: every board is evaluated and
+    matches the same board evaluated alone."""
+    import leela_amd
+    planes = synthetic_planes(1100, seed=11)
+    raw, prob = gpu.forward(leela_amd.NET_POLICY, leela_amd.PREC_BF16, planes)
+    assert np.isfinite(raw).all()
+    np.testing.assert_allclose(prob.sum(1), 1.0, atol=1e-5)
+    one, _ = gpu.forward(leela_amd.NET_POLICY, leela_amd.PREC_BF16,
+                         planes[[1099]])
+    assert np.array_equal(raw[[1099]], one)
+    v, w = gpu.forward(leela_amd.NET_VALUE, leela_amd.PREC_FP16, planes)
+    assert np.isfinite(v).all() and ((w >= 0) & (w <= 1)).all()

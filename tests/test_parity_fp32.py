@@ -4,7 +4,7 @@ within 1e-4 absolute, identical top-1 moves."""
 import numpy as np
 import pytest
 
-from tests.helpers import synthetic_planes
+from leela_amd.positions import make_batch, synthetic_planes
 
 pytestmark = pytest.mark.gpu
 
@@ -76,10 +76,11 @@ def test_fp32_rows_independent_of_batch(gpu):
 
 
 def test_fp32_config1_batch256_vs_cpu(gpu, cpu_nets):
-    """Batch 256: the plain form against the CPU path on every row."""
+    """Batch 256 of generator positions (seeds 1..256, up to 250 moves): the
+    plain form against the CPU path on every row."""
     import oracle
     import leela_amd
-    planes = synthetic_planes(256, seed=1)
+    planes = make_batch(256, seed0=1)
     raw_ref, _ = oracle.forward_f32(cpu_nets[0], planes, oracle.HEAD_POLICY)
     raw, _ = gpu.forward(leela_amd.NET_POLICY, leela_amd.PREC_FP32, planes)
     assert np.abs(raw - raw_ref).max() < TOL
