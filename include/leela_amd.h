@@ -98,6 +98,48 @@ int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
                      void *stream, float *launch_ms, int max_launches,
                      int *n_launches);
 
+/* ---- Batched leaf queue -------------------------------------------------
+ * Replaces the per-thread asynchronous path: Network::async_scored_moves
+ * -> OpenCL_Network::forward with forward_cb (Network.cpp:471-587,
+ * OpenCL.cpp:490-569), which evaluates one position per forward per search
+ * thread. Any number of threads submit single positions; a dispatcher
+ * thread batches up to max_batch of one net (or whatever has waited
+ * max_wait_us) and replays a hipGraph captured for the smallest bucket of
+ * 64 / 256 / 1024 / 2048 boards that holds the batch. The softmax
+ * temperature in effect at lz_queue_start is baked into the graphs. */
+
+/* Completion callback, run on the dispatcher thread. raw/out point at the
+ * position's results (policy: 361 floats each; value: 1 float each) and are
+ * valid only during the call; status != LZ_OK means the batch failed and
+ * raw/out are NULL. */
+typedef void (*lz_done_fn)(void *user, int status, const float *raw,
+                           const float *out);
+
+int lz_queue_start(lz_ctx *ctx, int precision, int max_batch,
+                   int max_wait_us);
+
+/* Copies planes (uint8 [32][361]) into the queue; `done` runs once with
+ * the result. Thread-safe. */
+int lz_submit(lz_ctx *ctx, int net, const uint8_t *planes, lz_done_fn done,
+              void *user);
+
+/* Launches what is queued without waiting for a full batch. */
+int lz_flush(lz_ctx *ctx);
+
+/* Flushes and waits until every submitted position's callback has run
+ * (OpenCL::join_outstanding_cb, OpenCL.cpp:571-577). */
+int lz_join(lz_ctx *ctx);
+
+/* Positions submitted whose callbacks have not run yet; the search's
+ * backpressure test (OpenCL::thread_can_issue, OpenCL.cpp:446-454). */
+int lz_queue_depth(lz_ctx *ctx);
+
+/* stats[6] = {batches, positions, batches per bucket 64/256/1024/2048}. */
+int lz_queue_stats(lz_ctx *ctx, int64_t *stats);
+
+/* Drains and stops the queue (also done by lz_destroy). */
+int lz_queue_stop(lz_ctx *ctx);
+
 /* Fills out[0..n) with N(0, stddev^2) draws: Box-Muller over the
  * reference's xoroshiro128+ (Random.cpp:27-57) seeded with `seed`. The
  * synthetic He-normal weight sets are made with it because the shipped
@@ -155,6 +197,35 @@ int lz_board_queries(const lz_position *pos, int32_t *out);
  * every hardware thread. */
 int lz_make_batch(int n, int side, uint32_t seed0, int max_moves, int net,
                   int threads, uint8_t *planes);
+
+/* ---- Network facade ----------------------------------------------------
+ * The evaluation API UCTSearch calls, with the reference's Netresult
+ * semantics, computed by this backend (C++ class lz::Network in
+ * leela_amd/csrc/network.h; these are its C entry points). */
+
+/* Network::get_scored_moves (Network.cpp:624-674): ensemble 0 DIRECT (with
+ * rotation 0..7), 1 RANDOM_ROTATION, 2 AVERAGE_ALL (eight symmetries in one
+ * batch). Writes the empty points' probabilities and vertices in row-major
+ * order (at most 361), losing-ladder escapes zeroed, not renormalised;
+ * *count = 0 for boards other than 19x19. */
+int lz_network_scored_moves(lz_ctx *ctx, int precision,
+                            const lz_position *pos, int ensemble,
+                            int rotation, float *probs, int32_t *vertices,
+                            int *count);
+
+/* Network::get_value (Network.cpp:590-622): win probability for the side
+ * to move (0.5 for boards other than 19x19). */
+int lz_network_value(lz_ctx *ctx, int precision, const lz_position *pos,
+                     int ensemble, float *value);
+
+/* Network::async_scored_moves (Network.cpp:536-587) with DIRECT rotation,
+ * through the leaf queue (lz_queue_start first; the queue's precision
+ * applies). `done` gets the Netresult on the dispatcher thread, as
+ * forward_cb does (Network.cpp:484-534). */
+typedef void (*lz_moves_fn)(void *user, int count, const float *probs,
+                            const int32_t *vertices);
+int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
+                                  int rotation, lz_moves_fn done, void *user);
 
 /* Device name, as OpenCL::get_device_name (OpenCL.h:121). */
 const char *lz_device_name(lz_ctx *ctx);

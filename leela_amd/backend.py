@@ -38,6 +38,10 @@ class LeelaError(RuntimeError):
 
 _lib = None
 
+# void (*lz_done_fn)(void *user, int status, const float *raw, const float *out)
+DONE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_float),
+                      C.POINTER(C.c_float))
+
 
 def load_library():
     """Load libleela_amd.so. Raises when it has not been built."""
@@ -64,6 +68,13 @@ def load_library():
                                     vp]
     L.lz_forward_timed.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
                                    vp, f32p, C.c_int, C.POINTER(C.c_int)]
+    L.lz_queue_start.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.lz_submit.argtypes = [vp, C.c_int, C.POINTER(C.c_uint8), DONE_FN, vp]
+    L.lz_flush.argtypes = [vp]
+    L.lz_join.argtypes = [vp]
+    L.lz_queue_depth.argtypes = [vp]
+    L.lz_queue_stats.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.lz_queue_stop.argtypes = [vp]
     L.lz_normal_fill.argtypes = [C.c_uint32, C.c_size_t, C.c_float, f32p]
     L.lz_device_name.argtypes = [vp]
     L.lz_device_name.restype = C.c_char_p
@@ -173,6 +184,43 @@ class Backend:
                                              _ptr(ms), 64, C.byref(cnt)),
                     "lz_forward_timed")
         return ms[:cnt.value].copy()
+
+    # ---- batched leaf queue (lz_queue_start / lz_submit / lz_join) ----
+
+    def queue_start(self, precision: int, max_batch: int = 1024,
+                    max_wait_us: int = 200):
+        """Starts the dispatcher. Results are delivered to the callbacks
+        given to submit(); keep the callback objects alive until join()."""
+        self._check(self._L.lz_queue_start(self._ctx, precision, max_batch,
+                                           max_wait_us), "lz_queue_start")
+
+    def submit(self, net: int, planes: np.ndarray, done, user: int = 0):
+        """planes uint8 [32][361]; done is a DONE_FN-wrapped callable."""
+        planes = np.ascontiguousarray(planes, dtype=np.uint8)
+        rc = self._L.lz_submit(self._ctx, net, _ptr(planes, C.c_uint8), done,
+                               C.c_void_p(user))
+        if rc != LZ_OK:
+            raise LeelaError(rc, "lz_submit")
+
+    def flush(self):
+        self._check(self._L.lz_flush(self._ctx), "lz_flush")
+
+    def join(self):
+        self._check(self._L.lz_join(self._ctx), "lz_join")
+
+    def queue_depth(self) -> int:
+        return int(self._L.lz_queue_depth(self._ctx))
+
+    def queue_stats(self) -> dict:
+        s = np.zeros(6, np.int64)
+        self._check(self._L.lz_queue_stats(self._ctx, _ptr(s, C.c_int64)),
+                    "lz_queue_stats")
+        return {"batches": int(s[0]), "positions": int(s[1]),
+                "per_bucket": {b: int(c) for b, c in
+                               zip((64, 256, 1024, 2048), s[2:])}}
+
+    def queue_stop(self):
+        self._check(self._L.lz_queue_stop(self._ctx), "lz_queue_stop")
 
     def close(self):
         if self._ctx:

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "lz_internal.h"
+#include "lz_ctx.h"
 #include "lz_net.h"
 
 namespace {
@@ -33,27 +34,32 @@ void set_init_err(const std::string &s) {
 
 }  // namespace
 
-struct lz_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string name;
-    std::string err;
-    std::mutex mu;
-    lz::Net net[2];
-    lz::Workspace ws;
-    float temperature = 0.75f;
-    bool finalized = false;
+namespace lz {
 
-    int fail(int code, const char *fmt, ...) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
-        va_end(ap);
-        err = buf;
-        return code;
-    }
-};
+int forward_locked(lz_ctx *ctx, int net, int precision, int n,
+                   const uint8_t *d_planes, float *d_raw, float *d_out,
+                   hipStream_t stream, LaunchTimer *timer) {
+    if (!ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return ctx->fail(LZ_ERR_ARG, "bad net id %d", net);
+    if (precision < LZ_PREC_FP32 || precision > LZ_PREC_FP16)
+        return ctx->fail(LZ_ERR_ARG, "bad precision %d", precision);
+    if (n < 0) return ctx->fail(LZ_ERR_ARG, "negative batch");
+    if (n == 0) return LZ_OK;
+    if (!d_planes) return ctx->fail(LZ_ERR_ARG, "planes is NULL");
+    std::string msg;
+    int rc = ensure_workspace(ctx->ws, ctx->net, n, &msg);
+    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    rc = run_net(ctx->net[net], net, precision, n, d_planes, d_raw, d_out,
+                 ctx->temperature, ctx->ws, stream, &msg, timer);
+    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    return LZ_OK;
+}
+
+}  // namespace lz
+
+using lz::forward_locked;
 
 extern "C" {
 
@@ -148,7 +154,7 @@ int lz_finalize(lz_ctx *ctx) {
     if (!ctx) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     if (ctx->finalized) return ctx->fail(LZ_ERR_STATE, "already finalized");
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     for (int k = 0; k < 2; k++) {
         std::string msg;
         int rc = lz::validate_net(ctx->net[k], k, &msg);
@@ -174,35 +180,13 @@ int lz_set_softmax_temp(lz_ctx *ctx, float temperature) {
     return LZ_OK;
 }
 
-static int forward_locked(lz_ctx *ctx, int net, int precision, int n,
-                          const uint8_t *d_planes, float *d_raw, float *d_out,
-                          hipStream_t stream,
-                          lz::LaunchTimer *timer = nullptr) {
-    if (!ctx->finalized)
-        return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
-    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
-        return ctx->fail(LZ_ERR_ARG, "bad net id %d", net);
-    if (precision < LZ_PREC_FP32 || precision > LZ_PREC_FP16)
-        return ctx->fail(LZ_ERR_ARG, "bad precision %d", precision);
-    if (n < 0) return ctx->fail(LZ_ERR_ARG, "negative batch");
-    if (n == 0) return LZ_OK;
-    if (!d_planes) return ctx->fail(LZ_ERR_ARG, "planes is NULL");
-    std::string msg;
-    int rc = lz::ensure_workspace(ctx->ws, ctx->net, n, &msg);
-    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
-    rc = lz::run_net(ctx->net[net], net, precision, n, d_planes, d_raw, d_out,
-                     ctx->temperature, ctx->ws, stream, &msg, timer);
-    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
-    return LZ_OK;
-}
-
 int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
                      const uint8_t *d_planes, float *d_raw, float *d_out,
                      void *stream, float *launch_ms, int max_launches,
                      int *n_launches) {
     if (!ctx) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     lz::LaunchTimer timer;
     int rc = forward_locked(ctx, net, precision, n, d_planes, d_raw, d_out, s,
@@ -228,7 +212,7 @@ int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
                       void *stream) {
     if (!ctx) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     return forward_locked(ctx, net, precision, n, d_planes, d_raw, d_out, s);
 }
@@ -242,7 +226,7 @@ int lz_forward(lz_ctx *ctx, int net, int precision, int n,
     if (!planes) return ctx->fail(LZ_ERR_ARG, "planes is NULL");
     if (!ctx->finalized)
         return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::string msg;
     int rc = lz::ensure_workspace(ctx->ws, ctx->net, n, &msg);
     if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
@@ -289,12 +273,13 @@ const char *lz_last_error(lz_ctx *ctx) {
 
 void lz_destroy(lz_ctx *ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    lz::destroy_queue(ctx);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     lz::free_net(ctx->net[0]);
     lz::free_net(ctx->net[1]);
     lz::free_workspace(ctx->ws);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 

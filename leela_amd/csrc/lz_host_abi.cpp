@@ -7,13 +7,10 @@
 #include <thread>
 #include <vector>
 
-#include "input_planes.h"
 #include "go_board.h"
+#include "input_planes.h"
 #include "leela_amd.h"
-
-struct lz_position {
-    lz::go::GameState st;
-};
+#include "lz_position.h"
 
 namespace {
 

@@ -1,0 +1,246 @@
+// network.cpp — the Network facade (Network.cpp:536-832 semantics) over
+// the C ABI, plus its C entry points.
+#include "network.h"
+
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <memory>
+#include <thread>
+
+#include "input_planes.h"
+#include "lz_position.h"
+#include "lz_random.h"
+
+namespace lz {
+namespace {
+
+constexpr int kP = 361;
+constexpr int kPos = kPlanes * kP;
+
+// Per-thread generator for RANDOM_ROTATION, seeded from the clock and the
+// thread id as Random::get_Rng does (Random.cpp:9-22).
+Rng &thread_rng() {
+    thread_local Rng rng(
+        (uint32_t)std::chrono::steady_clock::now().time_since_epoch().count() ^
+        (uint32_t)std::hash<std::thread::id>()(std::this_thread::get_id()));
+    return rng;
+}
+
+int pick_rotation(Network::Ensemble e, int rotation) {
+    return e == Network::RANDOM_ROTATION ? (int)thread_rng().randfix<8>()
+                                         : rotation;
+}
+
+}  // namespace
+
+int Network::rotate_nn_idx(int vertex, int symmetry) {
+    return lz::rotate_nn_idx(vertex, symmetry, 19);
+}
+
+int Network::rev_rotate_nn_idx(int vertex, int symmetry) {
+    return lz::rev_rotate_nn_idx(vertex, symmetry, 19);
+}
+
+void Network::softmax(const std::vector<float> &input,
+                      std::vector<float> &output, float temperature) {
+    float alpha = input[0];
+    for (size_t i = 1; i < output.size(); i++) alpha = std::max(alpha, input[i]);
+    alpha /= temperature;
+    float denom = 0.0f;
+    std::vector<float> helper(output.size());
+    for (size_t i = 0; i < output.size(); i++) {
+        helper[i] = std::exp(input[i] / temperature - alpha);
+        denom += helper[i];
+    }
+    for (size_t i = 0; i < output.size(); i++) output[i] = helper[i] / denom;
+}
+
+void collect_scored_moves(const go::GameState &state, const float *probs,
+                          int rotation, Network::Netresult &out) {
+    const go::Board &b = state.board;
+    out.clear();
+    for (int idx = 0; idx < kP; idx++) {
+        const int x = idx % 19, y = idx / 19;
+        const int v = b.vertex(x, y);
+        if (b.at(v) == go::EMPTY)
+            out.emplace_back(probs[Network::rev_rotate_nn_idx(idx, rotation)], v);
+    }
+}
+
+void prune_ladders(const go::GameState &state, const uint8_t *ladder,
+                   Network::Netresult &result) {
+    for (auto &sm : result) {
+        const auto xy = state.board.xy(sm.second);
+        if (ladder[xy.second * 19 + xy.first]) sm.first = 0.0f;
+    }
+}
+
+Network::Netresult Network::get_scored_moves(const go::GameState &state,
+                                             Ensemble ensemble,
+                                             int rotation) {
+    Netresult result;
+    status_ = LZ_OK;
+    if (state.board.side() != 19) return result;
+    std::vector<uint8_t> planes(kPos);
+    gather_features_policy(state, planes.data());
+    const uint8_t *ladder = planes.data() + 25 * kP;
+
+    const int rows = ensemble == AVERAGE_ALL ? 8 : 1;
+    std::vector<int> rots(rows);
+    for (int r = 0; r < rows; r++)
+        rots[r] = ensemble == AVERAGE_ALL ? r : pick_rotation(ensemble, rotation);
+    if (rots[0] < 0 || rots[0] > 7) {
+        status_ = LZ_ERR_ARG;
+        return result;
+    }
+    std::vector<uint8_t> in((size_t)rows * kPos);
+    for (int r = 0; r < rows; r++)
+        rotate_planes(planes.data(), rots[r], 19, in.data() + (size_t)r * kPos);
+    std::vector<float> probs((size_t)rows * kP);
+    status_ = lz_forward(ctx_, LZ_NET_POLICY, precision_, rows, in.data(),
+                         nullptr, probs.data());
+    if (status_ != LZ_OK) return result;
+
+    collect_scored_moves(state, probs.data(), rots[0], result);
+    if (rows == 8) {
+        Netresult other;
+        for (int r = 1; r < 8; r++) {
+            collect_scored_moves(state, probs.data() + (size_t)r * kP, rots[r],
+                                 other);
+            for (size_t i = 0; i < result.size(); i++)
+                result[i].first += other[i].first;
+        }
+        for (auto &sn : result) sn.first /= 8.0f;
+    }
+    prune_ladders(state, ladder, result);
+    return result;
+}
+
+float Network::get_value(const go::GameState &state, Ensemble ensemble) {
+    status_ = LZ_OK;
+    if (state.board.side() != 19) return 0.5f;
+    std::vector<uint8_t> planes(kPos);
+    gather_features_value(state, planes.data());
+    const int rows = ensemble == AVERAGE_ALL ? 8 : 1;
+    std::vector<uint8_t> in((size_t)rows * kPos);
+    for (int r = 0; r < rows; r++) {
+        const int rot = ensemble == AVERAGE_ALL ? r
+                        : ensemble == DIRECT    ? 0
+                                                : pick_rotation(ensemble, -1);
+        rotate_planes(planes.data(), rot, 19, in.data() + (size_t)r * kPos);
+    }
+    std::vector<float> win(rows);
+    status_ = lz_forward(ctx_, LZ_NET_VALUE, precision_, rows, in.data(),
+                         nullptr, win.data());
+    if (status_ != LZ_OK) return 0.5f;
+    float sum = 0.0f;
+    for (float w : win) sum += w;
+    return sum / rows;
+}
+
+namespace {
+
+// What forward_cb needs after the batch (Network.cpp:472-482): a copy of
+// the state, the rotation, the ladder plane and the consumer's callback.
+struct AsyncJob {
+    go::GameState state;
+    int rotation;
+    std::vector<uint8_t> ladder;
+    Network::ScoringCallback cb;
+};
+
+void async_done(void *user, int status, const float *raw, const float *out) {
+    (void)raw;
+    std::unique_ptr<AsyncJob> job(static_cast<AsyncJob *>(user));
+    Network::Netresult result;
+    if (status == LZ_OK) {
+        collect_scored_moves(job->state, out, job->rotation, result);
+        prune_ladders(job->state, job->ladder.data(), result);
+    }
+    job->cb(result);
+}
+
+}  // namespace
+
+int Network::async_scored_moves(const go::GameState &state,
+                                ScoringCallback cb, Ensemble ensemble,
+                                int rotation) {
+    if (state.board.side() != 19) return LZ_OK;  // as Network.cpp:541-543
+    if (ensemble == AVERAGE_ALL) return LZ_ERR_ARG;
+    const int rot = pick_rotation(ensemble, rotation);
+    if (rot < 0 || rot > 7) return LZ_ERR_ARG;
+    std::vector<uint8_t> planes(kPos), in(kPos);
+    gather_features_policy(state, planes.data());
+    rotate_planes(planes.data(), rot, 19, in.data());
+    auto job = std::make_unique<AsyncJob>();
+    job->state = state;
+    job->rotation = rot;
+    job->ladder.assign(planes.begin() + 25 * kP, planes.begin() + 26 * kP);
+    job->cb = std::move(cb);
+    const int rc =
+        lz_submit(ctx_, LZ_NET_POLICY, in.data(), async_done, job.get());
+    if (rc == LZ_OK) job.release();
+    return rc;
+}
+
+bool Network::can_issue(int max_outstanding) const {
+    return lz_queue_depth(ctx_) < max_outstanding;
+}
+
+void Network::join_outstanding_cb() { lz_join(ctx_); }
+
+}  // namespace lz
+
+extern "C" {
+
+int lz_network_scored_moves(lz_ctx *ctx, int precision,
+                            const lz_position *pos, int ensemble,
+                            int rotation, float *probs, int32_t *vertices,
+                            int *count) {
+    if (!ctx || !pos || !probs || !vertices || !count || ensemble < 0 ||
+        ensemble > 2)
+        return LZ_ERR_ARG;
+    lz::Network net(ctx, precision);
+    const auto res = net.get_scored_moves(
+        pos->st, static_cast<lz::Network::Ensemble>(ensemble), rotation);
+    if (net.last_status() != LZ_OK) return net.last_status();
+    *count = (int)res.size();
+    for (size_t i = 0; i < res.size(); i++) {
+        probs[i] = res[i].first;
+        vertices[i] = res[i].second;
+    }
+    return LZ_OK;
+}
+
+int lz_network_value(lz_ctx *ctx, int precision, const lz_position *pos,
+                     int ensemble, float *value) {
+    if (!ctx || !pos || !value || ensemble < 0 || ensemble > 2)
+        return LZ_ERR_ARG;
+    lz::Network net(ctx, precision);
+    *value = net.get_value(pos->st,
+                           static_cast<lz::Network::Ensemble>(ensemble));
+    return net.last_status();
+}
+
+int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
+                                  int rotation, lz_moves_fn done,
+                                  void *user) {
+    if (!ctx || !pos || !done || rotation < 0 || rotation > 7)
+        return LZ_ERR_ARG;
+    lz::Network net(ctx, LZ_PREC_FP32);  // the queue fixes the precision
+    return net.async_scored_moves(
+        pos->st,
+        [done, user](const lz::Network::Netresult &r) {
+            std::vector<float> p(r.size());
+            std::vector<int32_t> v(r.size());
+            for (size_t i = 0; i < r.size(); i++) {
+                p[i] = r[i].first;
+                v[i] = r[i].second;
+            }
+            done(user, (int)r.size(), p.data(), v.data());
+        },
+        lz::Network::DIRECT, rotation);
+}
+
+}  // extern "C"

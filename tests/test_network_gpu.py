@@ -1,0 +1,172 @@
+"""GPU tests of the Network facade: get_scored_moves / get_value /
+async_scored_moves against the reference semantics (Network.cpp:536-832)
+restated here from the raw batched forward, and the FP32 drop-in against
+the oracle's reference CPU path end to end."""
+import ctypes as C
+import threading
+
+import numpy as np
+import pytest
+
+from leela_amd.positions import NET_POLICY, NET_VALUE, Position
+
+pytestmark = pytest.mark.gpu
+
+INVERSE = [0, 1, 2, 3, 4, 6, 5, 7]
+
+
+def rot(idx, s):
+    x, y = idx % 19, idx // 19
+    if s >= 4:
+        x, y = y, x
+    if s % 4 in (1, 3):
+        y = 18 - y
+    if s % 4 in (2, 3):
+        x = 18 - x
+    return y * 19 + x
+
+
+def expected_moves(pos, probs_by_rot, rots):
+    """Network.cpp:820-829 + 645-667: reverse-rotate each symmetry's output,
+    keep empty points in row-major order, average, zero ladder escapes."""
+    sq = pos.squares()
+    ladder = pos.planes(NET_POLICY, 0)[25]
+    empt = [i for i in range(361) if sq[i] == 2]
+    acc = None
+    for probs, r in zip(probs_by_rot, rots):
+        vals = np.array([probs[rot(i, INVERSE[r])] for i in empt], np.float32)
+        acc = vals if acc is None else (acc + vals).astype(np.float32)
+    if len(rots) == 8:
+        acc = (acc / np.float32(8)).astype(np.float32)
+    acc = np.where(ladder[empt] == 1, np.float32(0), acc).astype(np.float32)
+    verts = np.array([(i // 19 + 1) * 21 + i % 19 + 1 for i in empt], np.int32)
+    return acc, verts
+
+
+@pytest.fixture(scope="module")
+def be(weights_b):
+    import leela_amd
+    b = leela_amd.Backend(0)
+    pol, val = weights_b
+    b.load_net(leela_amd.NET_POLICY, pol)
+    b.load_net(leela_amd.NET_VALUE, val)
+    b.finalize()
+    yield b
+    b.close()
+
+
+def ladder_positions():
+    """Generator positions that contain a losing-ladder escape point."""
+    found = []
+    for seed in range(1, 400):
+        p = Position.random(seed, 19, 250)
+        if p.planes(NET_POLICY)[25].any():
+            found.append(p)
+        if len(found) == 3:
+            break
+    return found
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "fp16"])
+def test_scored_moves_direct_and_average(be, prec_name):
+    import leela_amd
+    from leela_amd.network import AVERAGE_ALL, DIRECT, Network
+    prec = leela_amd.PRECISIONS[prec_name]
+    net = Network(be, prec)
+    poss = [Position.random(s, 19, 250) for s in (3, 17, 40)] + ladder_positions()
+    assert any(p.planes(NET_POLICY)[25].any() for p in poss)
+    for pos in poss:
+        rows = np.stack([pos.planes(NET_POLICY, r) for r in range(8)])
+        _, probs8 = be.forward(leela_amd.NET_POLICY, prec, rows)
+        for r in (0, 5):
+            got_p, got_v = net.get_scored_moves(pos, DIRECT, r)
+            want_p, want_v = expected_moves(pos, [probs8[r]], [r])
+            assert np.array_equal(got_v, want_v)
+            assert np.array_equal(got_p, want_p)
+        got_p, got_v = net.get_scored_moves(pos, AVERAGE_ALL)
+        want_p, want_v = expected_moves(pos, list(probs8), list(range(8)))
+        assert np.array_equal(got_v, want_v)
+        np.testing.assert_allclose(got_p, want_p, rtol=0, atol=1e-7)
+        assert got_p.sum() <= 1.0 + 1e-5
+        assert np.all(np.diff(got_v) > 0)
+
+
+def test_fp32_facade_matches_cpu_path(be, weights_b):
+    """The drop-in end to end: FP32 get_scored_moves / get_value against the
+    oracle's reference CPU path on the same encoded planes."""
+    import leela_amd
+    import oracle
+    from leela_amd.network import DIRECT, Network
+    pol, val = weights_b
+    oracle.load_blas()
+    net = Network(be, leela_amd.PREC_FP32)
+    for seed in (8, 21, 99):
+        pos = Position.random(seed, 19, 250)
+        for r in (0, 6):
+            pl = pos.planes(NET_POLICY, r)[None]
+            _, cpu = oracle.forward_f32(oracle.Net(pol), pl, oracle.HEAD_POLICY)
+            want_p, want_v = expected_moves(pos, [cpu[0]], [r])
+            got_p, got_v = net.get_scored_moves(pos, DIRECT, r)
+            assert np.array_equal(got_v, want_v)
+            assert np.abs(got_p - want_p).max() < 1e-4
+            if want_p.size:
+                assert got_v[got_p.argmax()] == want_v[want_p.argmax()]
+        vpl = pos.planes(NET_VALUE, 0)[None]
+        _, vw = oracle.forward_f32(oracle.Net(val), vpl, oracle.HEAD_VALUE)
+        assert abs(net.get_value(pos, DIRECT) - vw[0, 0]) < 1e-4
+
+
+def test_value_ensembles(be):
+    import leela_amd
+    from leela_amd.network import AVERAGE_ALL, DIRECT, RANDOM_ROTATION, Network
+    net = Network(be, leela_amd.PREC_FP32)
+    pos = Position.random(77, 19, 250)
+    rows = np.stack([pos.planes(NET_VALUE, r) for r in range(8)])
+    _, w = be.forward(leela_amd.NET_VALUE, leela_amd.PREC_FP32, rows)
+    assert net.get_value(pos, DIRECT) == w[0, 0]
+    assert abs(net.get_value(pos, AVERAGE_ALL) - w[:, 0].mean()) < 1e-6
+    assert any(abs(net.get_value(pos, RANDOM_ROTATION) - w[r, 0]) < 1e-7
+               for r in range(8))
+
+
+def test_non_19_boards(be):
+    import leela_amd
+    from leela_amd.network import DIRECT, Network
+    net = Network(be, leela_amd.PREC_FP32)
+    pos = Position.random(5, 13, 101)
+    p, v = net.get_scored_moves(pos, DIRECT, 0)
+    assert p.size == 0 and v.size == 0
+    assert net.get_value(pos, DIRECT) == 0.5
+
+
+def test_async_scored_moves_through_queue(be):
+    import leela_amd
+    from leela_amd.network import DIRECT, MOVES_FN, Network
+    net = Network(be, leela_amd.PREC_FP32)
+    poss = [Position.random(s, 19, 250) for s in range(600, 664)]
+    want = [net.get_scored_moves(p, DIRECT, i % 8) for i, p in enumerate(poss)]
+    got = [None] * len(poss)
+
+    def done(user, count, probs, verts):
+        i = int(user or 0)
+        got[i] = (np.ctypeslib.as_array(probs, (count,)).copy() if count else
+                  np.zeros(0, np.float32),
+                  np.ctypeslib.as_array(verts, (count,)).copy() if count else
+                  np.zeros(0, np.int32))
+    cb = MOVES_FN(done)
+    be.queue_start(leela_amd.PREC_FP32, max_batch=64, max_wait_us=500)
+    try:
+        def producer(t):
+            for i in range(t, len(poss), 16):
+                net.async_scored_moves(poss[i], i % 8, cb, i)
+        ths = [threading.Thread(target=producer, args=(t,)) for t in range(16)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        be.join()
+    finally:
+        be.queue_stop()
+    for i in range(len(poss)):
+        assert np.array_equal(got[i][1], want[i][1])
+        assert np.array_equal(got[i][0], want[i][0])
