@@ -88,6 +88,23 @@ int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
                       const uint8_t *d_planes, float *d_raw, float *d_out,
                       void *stream);
 
+/* Ragged batch: n boards of sides[i] in 5..19 (e.g. mixed 9x9 / 13x13 /
+ * 19x19) in one persistent launch of the 16-bit fused kernels
+ * (LZ_PREC_BF16 / LZ_PREC_FP16; FP32 returns LZ_ERR_UNSUPPORTED). Planes
+ * are packed board after board, uint8 [32][S_i*S_i] each; policy raw/out
+ * likewise (S_i*S_i floats per board), value one float per board. The
+ * reference nets are 19x19 only (Network.cpp:541-543); for smaller boards
+ * the convs are the same same-padded layers and the value head's 361-input
+ * inner product sees the S x S plane at y*19+x with zeros elsewhere.
+ * Same role as lz_forward: replaces OpenCL_Network::forward. */
+int lz_forward_ragged(lz_ctx *ctx, int net, int precision, int n,
+                      const int32_t *sides, const uint8_t *planes, float *raw,
+                      float *out);
+/* Device-buffer form (sides stays a host array), enqueued on `stream`. */
+int lz_forward_ragged_device(lz_ctx *ctx, int net, int precision, int n,
+                             const int32_t *sides, const uint8_t *d_planes,
+                             float *d_raw, float *d_out, void *stream);
+
 /* As lz_forward_device, and also times every kernel launch of the forward
  * with HIP events recorded on `stream` between launches. Writes up to
  * max_launches durations (ms, launch order) to launch_ms and the launch

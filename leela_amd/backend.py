@@ -68,6 +68,12 @@ def load_library():
                                     vp]
     L.lz_forward_timed.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
                                    vp, f32p, C.c_int, C.POINTER(C.c_int)]
+    L.lz_forward_ragged.argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_uint8), f32p, f32p]
+    L.lz_forward_ragged_device.argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(C.c_int32), vp, vp, vp,
+                                           vp]
     L.lz_queue_start.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.lz_submit.argtypes = [vp, C.c_int, C.POINTER(C.c_uint8), DONE_FN, vp]
     L.lz_flush.argtypes = [vp]
@@ -184,6 +190,32 @@ class Backend:
                                              _ptr(ms), 64, C.byref(cnt)),
                     "lz_forward_timed")
         return ms[:cnt.value].copy()
+
+    def forward_ragged(self, net: int, precision: int, boards):
+        """boards: list of uint8 [32][S*S] plane stacks of any sides 5..19.
+        Returns per-board (raw, out) arrays (policy S*S each, value 1)."""
+        sides = np.array([int(round(np.sqrt(b.shape[-1]))) for b in boards],
+                         np.int32)
+        flat = np.concatenate([np.ascontiguousarray(b, np.uint8).ravel()
+                               for b in boards])
+        per = [s * s if net == NET_POLICY else 1 for s in sides]
+        tot = int(sum(per))
+        raw = np.zeros(tot, np.float32)
+        out = np.zeros(tot, np.float32)
+        self._check(self._L.lz_forward_ragged(
+            self._ctx, net, precision, len(boards), _ptr(sides, C.c_int32),
+            _ptr(flat, C.c_uint8), _ptr(raw), _ptr(out)), "lz_forward_ragged")
+        offs = np.concatenate([[0], np.cumsum(per)])
+        return ([raw[offs[i]:offs[i + 1]] for i in range(len(boards))],
+                [out[offs[i]:offs[i + 1]] for i in range(len(boards))])
+
+    def forward_ragged_device(self, net: int, precision: int, sides,
+                              d_planes: int, d_raw: int, d_out: int,
+                              stream: int | None = None):
+        sides = np.ascontiguousarray(sides, np.int32)
+        self._check(self._L.lz_forward_ragged_device(
+            self._ctx, net, precision, len(sides), _ptr(sides, C.c_int32),
+            d_planes, d_raw, d_out, stream), "lz_forward_ragged_device")
 
     # ---- batched leaf queue (lz_queue_start / lz_submit / lz_join) ----
 

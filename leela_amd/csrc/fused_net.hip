@@ -1,29 +1,32 @@
 // fused_net.hip — persistent fused layer-chain kernel: all convs of the
-// policy (or value) net for one board per workgroup pass, 16-bit operands
-// (BF16, or FP16) on MFMA with FP32 accumulation, activations resident in
-// LDS.
+// policy (or value) net for one slot of boards per workgroup pass, 16-bit
+// operands (BF16 or FP16) on MFMA with FP32 accumulation, activations
+// resident in LDS.
 //
 // Work decomposition
-//   * A workgroup owns one board at a time and loops over boards
-//     (persistent grid). The board's activations live in LDS as 16-bit
-//     values in an HWC tile with a zero halo: [SP][SP][PS], SP = 19 + 2*pad,
-//     PS = C + 8 (the 8-element pad staggers pixels across LDS banks).
-//   * Each conv is an implicit GEMM D[cout x 361] = W[cout x K] B[K x 361],
-//     K = k*k*cin ordered tap-major, 32 channels per MFMA k-step
+//   * A workgroup owns one slot at a time and loops over slots (persistent
+//     grid). A slot is one 19x19 board, or up to four smaller boards of one
+//     side (two 13x13, four 9x9: ragged batches). The slot's activations
+//     live in LDS as 16-bit values in an HWC tile with zero halos:
+//     boards are stacked vertically with one zero separator row between
+//     them, row pitch S + 2, pixel stride PS = C + 8 (the 8-element pad
+//     staggers pixels across LDS banks).
+//   * Each conv is an implicit GEMM D[cout x pixels] = W[cout x K] B[K x
+//     pixels], K = k*k*cin ordered tap-major, 32 channels per MFMA k-step
 //     (v_mfma_f32_16x16x32_{bf16,f16}). The waves form a WM (output
 //     channels) x 4 (pixels) grid (policy 2 x 4, value 1 x 4); a wave owns
 //     1/WM of the output channels of six 16-pixel column tiles
-//     (4 x 6 x 16 = 384 >= 361 pixels). Its B fragments are one 16-byte
-//     ds_read per lane at pixel_base + tap_offset + channel_block,
-//     prefetched one k-step ahead.
+//     (4 x 6 x 16 = 384 >= 361 pixels; two 13x13 boards are 338, four 9x9
+//     are 324). Its B fragments are one 16-byte ds_read per lane at
+//     pixel_base + tap_offset + channel_block, prefetched one k-step ahead.
 //   * Weights are pre-packed on the host in MFMA fragment order, [k-step]
 //     [m-tile][lane][8], so an A operand is one coalesced 16-byte global load
 //     per lane, prefetched one k-step ahead. The pixel-waves of a channel
 //     group walk the same k-steps together, so repeated fragment reads hit
-//     in L1 and each CU pulls a layer's weights from L2 once per board.
+//     in L1 and each CU pulls a layer's weights from L2 once per slot.
 //   * Epilogue: bias + ELU in FP32 (Network.cpp:382-392), rounded to the
 //     operand type, written back over the input tile after a barrier. The
-//     last 1-channel conv keeps FP32: softmax(T) over the logits
+//     last 1-channel conv keeps FP32: softmax(T) per board over its logits
 //     (Network.cpp:450-469) for the policy net; for the value net the ELU'd
 //     plane goes to HBM and the batched inner-product head (value_fc_head)
 //     finishes it.
@@ -58,28 +61,22 @@ constexpr int kWN = 4;                // waves along pixels
 constexpr int kNT = 6;                // 16-pixel tiles per wave
 constexpr int kP = 361;
 constexpr int kInPS = 40;             // pixel stride of the 32-plane input
+constexpr int kTilePixels = 456;      // LDS tile capacity in pixels
 
 __device__ __forceinline__ float elu(float v) {
     return v > 0.0f ? v : (expf(v) - 1.0f);
 }
 
-// Offset (in elements) of pixel j's window origin in a tile of side SP and
-// pixel stride PS. Padding columns (j >= 361) alias pixel 0.
-template <int SP, int PS>
-__device__ __forceinline__ int pix_base(int j) {
-    if (j >= kP) j = 0;
-    return ((j / 19) * SP + (j % 19)) * PS;
-}
-
-// One same-padded conv of the tile `act` (side SP, stride PS, CIN input
-// channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the packed
-// weights) for the wave's column tiles.
-template <typename T, int F, int CIN, int MT, int MTW, int SP, int PS>
+// One same-padded conv of the tile `act` (row pitch sp, stride PS, CIN
+// input channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the
+// packed weights) for the wave's column tiles; pb[t] is the lane's window
+// origin in column tile t.
+template <typename T, int F, int CIN, int MT, int MTW, int PS>
 __device__ __forceinline__ void conv_tile(const T *act,
                                           const typename Op<T>::v8 *__restrict__ wpk,
                                           f32x4 (&acc)[MTW][kNT],
-                                          const int (&pb)[kNT], int mt0,
-                                          int lane) {
+                                          const int (&pb)[kNT], int sp,
+                                          int mt0, int lane) {
     typedef typename Op<T>::v8 v8;
     constexpr int CB = CIN / 32;
     constexpr int STEPS = F * F * CB;
@@ -100,7 +97,7 @@ __device__ __forceinline__ void conv_tile(const T *act,
     for (int s = 0; s < STEPS; s++) {
         const int nxt = s + 1 < STEPS ? s + 1 : s;
         const int tap = nxt / CB, cb = nxt % CB;
-        const int off = ((tap / F) * SP + (tap % F)) * PS + cb * 32 + g8;
+        const int off = ((tap / F) * sp + (tap % F)) * PS + cb * 32 + g8;
         v8 bn[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++)
@@ -117,15 +114,16 @@ __device__ __forceinline__ void conv_tile(const T *act,
     }
 }
 
-// Bias + ELU of acc, written into the trunk tile (side 21, stride PSO).
-// D layout of 16x16 MFMA: lane holds rows 4*(lane/16)+r of column lane%16,
-// i.e. four consecutive output channels of one pixel.
-template <typename T, int MTW, int PSO>
+// Bias + ELU of acc, written into the trunk tile at the lane's pixel
+// offsets po[t] (-1: padding column). D layout of 16x16 MFMA: lane holds
+// rows 4*(lane/16)+r of column lane%16, i.e. four consecutive output
+// channels of one pixel.
+template <typename T, int MTW>
 __device__ __forceinline__ void store_tile(T *act,
                                            const f32x4 (&acc)[MTW][kNT],
                                            const float *__restrict__ bias,
-                                           int cout, int mt0, int nt0,
-                                           int lane) {
+                                           int cout, int mt0,
+                                           const int (&po)[kNT], int lane) {
     typedef typename Op<T>::v4 v4;
     const int g4 = (lane >> 4) * 4;
 #pragma unroll
@@ -136,13 +134,11 @@ __device__ __forceinline__ void store_tile(T *act,
         for (int r = 0; r < 4; r++) bv[r] = o0 + r < cout ? bias[o0 + r] : 0.0f;
 #pragma unroll
         for (int t = 0; t < kNT; t++) {
-            const int j = (nt0 + t) * 16 + (lane & 15);
-            if (j >= kP) continue;
+            if (po[t] < 0) continue;
             v4 v;
 #pragma unroll
             for (int r = 0; r < 4; r++) v[r] = (T)elu(acc[m][t][r] + bv[r]);
-            const int y = j / 19 + 1, x = j % 19 + 1;
-            *reinterpret_cast<v4 *>(act + (y * 21 + x) * PSO + o0) = v;
+            *reinterpret_cast<v4 *>(act + po[t] + o0) = v;
         }
     }
 }
@@ -171,29 +167,32 @@ __device__ float block_max(float v, float *s_red) {
 
 // T: operand type; C: trunk width (128 policy, 64 value); M1: outputs of
 // the 5x5 stem; NMID: 3x3 convs between stem and head (the first takes M1
-// inputs). Policy: writes raw = ELU'd logits and out = softmax(raw / T).
-// Value: writes the ELU'd head plane to h0 [n][361] for value_fc_head.
+// inputs). RAGGED: slots from args.slots (boards of any side up to 19,
+// planes and policy outputs at per-board offsets), else slot b = 19x19
+// board b with planes at 32*361*b and outputs at 361*b.
+// Policy: writes raw = ELU'd logits and out = softmax(raw / T) per board.
+// Value: writes each board's ELU'd head plane to h0 [n][361] (a smaller
+// board's y*S+x pixel at y*19+x, zeros elsewhere) for value_fc_head.
 // kWM waves split the output channels, kWN = 4 the pixels; OCC is the
 // waves per SIMD the register budget is sized for.
-template <typename T, int C, int M1, int NMID, bool VALUE, int kWM, int OCC>
+template <typename T, int C, int M1, int NMID, bool VALUE, bool RAGGED,
+          int kWM, int OCC>
 __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
     const uint8_t *__restrict__ planes, FusedArgs args,
-    float *__restrict__ raw, float *__restrict__ out, int n,
+    float *__restrict__ raw, float *__restrict__ out, int nslots,
     float temperature) {
     typedef typename Op<T>::v8 v8;
     constexpr int PS = C + 8;
-    constexpr int SP = 21;
-    constexpr int TILE = SP * SP * PS;         // trunk tile (elements)
-    constexpr int TILE0 = 23 * 23 * kInPS;     // stem input tile
-    constexpr int TILE_MAX = TILE > TILE0 ? TILE : TILE0;
+    constexpr int TILE = (RAGGED ? kTilePixels : 21 * 21) * PS;
     constexpr int kWaves = kWM * kWN;
     constexpr int kThreads = kWaves * 64;
     constexpr int MT = C / 16, MTW = MT / kWM;
     constexpr int MT1 = M1 / 16, MTW1 = MT1 / kWM;
     static_assert(C % 32 == 0 && M1 % 32 == 0 && MT % kWM == 0 &&
                       MT1 % kWM == 0, "channel blocking");
+    static_assert(23 * 23 * kInPS <= TILE, "stem tile fits the trunk tile");
 
-    __shared__ __attribute__((aligned(16))) T s_act[TILE_MAX];
+    __shared__ __attribute__((aligned(16))) T s_act[TILE];
     __shared__ float s_vec[384];
     __shared__ float s_red[kWaves];
 
@@ -202,25 +201,45 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
     const int nt0 = wn * kNT;
     auto W = [&](int l) { return static_cast<const v8 *>(args.w[l]); };
 
-    int pb_in[kNT], pb[kNT];
+    for (int slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+        // Slot geometry: k boards of side S stacked in the tile.
+        int S = 19, k = 1;
+        int board[4] = {slot, 0, 0, 0};
+        if (RAGGED) {
+            const FusedSlot sl = args.slots[slot];
+            S = sl.side;
+            k = sl.count;
 #pragma unroll
-    for (int t = 0; t < kNT; t++) {
-        const int j = (nt0 + t) * 16 + (lane & 15);
-        pb_in[t] = pix_base<23, kInPS>(j);
-        pb[t] = pix_base<SP, PS>(j);
-    }
+            for (int i = 0; i < 4; i++) board[i] = sl.board[i];
+        }
+        const int SS = S * S, npix = k * SS;
+        const int pin = S + 4, pt = S + 2;  // stem / trunk row pitch
+        int pb_in[kNT], pb[kNT], po[kNT];
+#pragma unroll
+        for (int t = 0; t < kNT; t++) {
+            const int j = (nt0 + t) * 16 + (lane & 15);
+            const int jj = j < npix ? j : 0;
+            const int kb = jj / SS, r = jj % SS, y = r / S, x = r % S;
+            pb_in[t] = ((kb * (S + 2) + y) * pin + x) * kInPS;
+            pb[t] = ((kb * (S + 1) + y) * pt + x) * PS;
+            po[t] = j < npix ? pb[t] + (pt + 1) * PS : -1;
+        }
 
-    for (int board = blockIdx.x; board < n; board += gridDim.x) {
         // Stage the 32 binary planes with a 2-wide zero halo.
-        const uint8_t *src = planes + (size_t)board * 32 * kP;
-        for (int i = tid; i < 23 * 23 * 4; i += kThreads) {
+        const int rows_in = 2 + k * (S + 2);
+        for (int i = tid; i < rows_in * pin * 4; i += kThreads) {
             const int pp = i >> 2, cg = (i & 3) * 8;
-            const int y = pp / 23 - 2, x = pp % 23 - 2;
-            const bool in = y >= 0 && y < 19 && x >= 0 && x < 19;
+            const int row = pp / pin - 2, xx = pp % pin - 2;
+            const int kb = row >= 0 ? row / (S + 2) : 0;
+            const int yy = row >= 0 ? row % (S + 2) : -1;
+            const bool in = row >= 0 && kb < k && yy < S && xx >= 0 && xx < S;
+            const uint8_t *src =
+                planes + (RAGGED ? (size_t)args.in_off[board[kb]]
+                                 : (size_t)board[0] * 32 * kP);
             v8 v;
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                v[e] = (T)(in ? (float)src[(cg + e) * kP + y * 19 + x] : 0.0f);
+                v[e] = (T)(in ? (float)src[(cg + e) * SS + yy * S + xx] : 0.0f);
             *reinterpret_cast<v8 *>(s_act + pp * kInPS + cg) = v;
         }
         __syncthreads();
@@ -228,133 +247,164 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         // Stem: 5x5, 32 -> M1.
         {
             f32x4 acc[MTW1][kNT];
-            conv_tile<T, 5, 32, MT1, MTW1, 23, kInPS>(s_act, W(0), acc, pb_in,
-                                                      wm * MTW1, lane);
+            conv_tile<T, 5, 32, MT1, MTW1, kInPS>(s_act, W(0), acc, pb_in,
+                                                  pin, wm * MTW1, lane);
             __syncthreads();
-            // Switch to the trunk geometry: clear tile and halo, then write.
-            for (int i = tid * 8; i < TILE; i += kThreads * 8)
+            // Switch to the trunk geometry: clear tile and halos, then write.
+            const int trunk = (1 + k * (S + 1)) * pt * PS;
+            for (int i = tid * 8; i < trunk; i += kThreads * 8)
                 *reinterpret_cast<v8 *>(s_act + i) = v8{};
             __syncthreads();
-            store_tile<T, MTW1, PS>(s_act, acc, args.b[0], M1, wm * MTW1, nt0,
-                                    lane);
+            store_tile<T, MTW1>(s_act, acc, args.b[0], M1, wm * MTW1, po,
+                                lane);
             __syncthreads();
         }
         // Middle 3x3 convs; the first reads M1 channels.
         {
             f32x4 acc[MTW][kNT];
-            conv_tile<T, 3, M1, MT, MTW, SP, PS>(s_act, W(1), acc, pb,
-                                                 wm * MTW, lane);
+            conv_tile<T, 3, M1, MT, MTW, PS>(s_act, W(1), acc, pb, pt,
+                                             wm * MTW, lane);
             __syncthreads();
-            store_tile<T, MTW, PS>(s_act, acc, args.b[1], C, wm * MTW, nt0,
-                                   lane);
+            store_tile<T, MTW>(s_act, acc, args.b[1], C, wm * MTW, po, lane);
             __syncthreads();
         }
         for (int l = 2; l <= NMID; l++) {
             f32x4 acc[MTW][kNT];
-            conv_tile<T, 3, C, MT, MTW, SP, PS>(s_act, W(l), acc, pb,
-                                                wm * MTW, lane);
+            conv_tile<T, 3, C, MT, MTW, PS>(s_act, W(l), acc, pb, pt,
+                                            wm * MTW, lane);
             __syncthreads();
-            store_tile<T, MTW, PS>(s_act, acc, args.b[l], C, wm * MTW, nt0,
-                                   lane);
+            store_tile<T, MTW>(s_act, acc, args.b[l], C, wm * MTW, po, lane);
             __syncthreads();
         }
         // Head: 3x3, C -> 1, in FP32; the channel-group-0 waves cover all
         // pixels.
         if (wm == 0) {
             f32x4 acc[1][kNT];
-            conv_tile<T, 3, C, 1, 1, SP, PS>(s_act, W(NMID + 1), acc, pb, 0,
-                                             lane);
+            conv_tile<T, 3, C, 1, 1, PS>(s_act, W(NMID + 1), acc, pb, pt, 0,
+                                         lane);
             const float hb = args.b[NMID + 1][0];
             if (lane < 16) {
 #pragma unroll
                 for (int t = 0; t < kNT; t++) {
                     const int j = (nt0 + t) * 16 + lane;
-                    if (j < kP) s_vec[j] = elu(acc[0][t][0] + hb);
+                    if (j < npix) s_vec[j] = elu(acc[0][t][0] + hb);
                 }
             }
         }
         __syncthreads();
-        if (VALUE) {
-            for (int p = tid; p < kP; p += kThreads)
-                args.h0[(size_t)board * kP + p] = s_vec[p];
-        } else {
-            float mx = -INFINITY;
-            for (int p = tid; p < kP; p += kThreads) mx = fmaxf(mx, s_vec[p]);
-            mx = block_max<kWaves>(mx, s_red);
-            const float alpha = mx / temperature;
-            float e[2] = {0.f, 0.f}, sum = 0.f;
-            for (int i = 0, p = tid; p < kP; p += kThreads, i++) {
-                e[i] = expf(s_vec[p] / temperature - alpha);
-                sum += e[i];
-            }
-            sum = block_sum<kWaves>(sum, s_red);
-            for (int i = 0, p = tid; p < kP; p += kThreads, i++) {
-                if (raw) raw[(size_t)board * kP + p] = s_vec[p];
-                if (out) out[(size_t)board * kP + p] = e[i] / sum;
+        for (int kb = 0; kb < k; kb++) {
+            const float *v = s_vec + kb * SS;
+            if (VALUE) {
+                float *h = args.h0 + (size_t)board[kb] * kP;
+                for (int p = tid; p < kP; p += kThreads) {
+                    const int y = p / 19, x = p % 19;
+                    h[p] = (y < S && x < S) ? v[y * S + x] : 0.0f;
+                }
+            } else {
+                const size_t o = RAGGED ? (size_t)args.out_off[board[kb]]
+                                        : (size_t)board[kb] * kP;
+                float mx = -INFINITY;
+                for (int p = tid; p < SS; p += kThreads) mx = fmaxf(mx, v[p]);
+                mx = block_max<kWaves>(mx, s_red);
+                const float alpha = mx / temperature;
+                float e[2] = {0.f, 0.f}, sum = 0.f;
+                for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
+                    e[i] = expf(v[p] / temperature - alpha);
+                    sum += e[i];
+                }
+                sum = block_sum<kWaves>(sum, s_red);
+                for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
+                    if (raw) raw[o + p] = v[p];
+                    if (out) out[o + p] = e[i] / sum;
+                }
             }
         }
         __syncthreads();
     }
 }
 
-// Policy: one 8-wave workgroup per CU (121 KB LDS tile), 2 waves per SIMD.
-// Value: 4-wave workgroups, two per CU (65 KB tile each), 2 waves per SIMD.
+// Policy: one 8-wave workgroup per CU (121-124 KB LDS tile), 2 waves per
+// SIMD. Value: 4-wave workgroups, two per CU (65 KB tile each), 2 waves per
+// SIMD.
 constexpr int kPolicyWM = 2, kValueWM = 1;
 constexpr int kPolicyThreads = kPolicyWM * kWN * 64;
 constexpr int kValueThreads = kValueWM * kWN * 64;
 
-template <typename T>
+template <typename T, bool RAGGED>
 struct Kernels {
-    static constexpr auto policy = fused_net<T, 128, 96, 11, false, kPolicyWM, 2>;
-    static constexpr auto value = fused_net<T, 64, 64, 10, true, kValueWM, 2>;
+    static constexpr auto policy =
+        fused_net<T, 128, 96, 11, false, RAGGED, kPolicyWM, 2>;
+    static constexpr auto value =
+        fused_net<T, 64, 64, 10, true, RAGGED, kValueWM, 2>;
 };
 
-template <typename T>
+template <typename T, bool RAGGED>
 hipError_t launch(int which, const uint8_t *planes, const FusedArgs &args,
-                  float *raw, float *out, int n, float temperature, int grid,
-                  hipStream_t stream) {
+                  float *raw, float *out, int nslots, int nboards,
+                  float temperature, int grid, hipStream_t stream) {
+    if (grid <= 0 || grid > nslots) grid = nslots;
     if (which == 0) {
         if (args.nconv != 13 || args.c != 128 || args.m1 != 96)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL(Kernels<T>::policy, dim3(grid),
+        hipLaunchKernelGGL((Kernels<T, RAGGED>::policy), dim3(grid),
                            dim3(kPolicyThreads), 0, stream, planes, args, raw,
-                           out, n, temperature);
+                           out, nslots, temperature);
         return hipGetLastError();
     }
     if (args.nconv != 12 || args.c != 64 || args.m1 != 64 || !args.h0)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(Kernels<T>::value, dim3(grid), dim3(kValueThreads), 0,
-                       stream, planes, args, raw, out, n, temperature);
+    hipLaunchKernelGGL((Kernels<T, RAGGED>::value), dim3(grid),
+                       dim3(kValueThreads), 0, stream, planes, args, raw, out,
+                       nslots, temperature);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return value_fc_head(args.h0, args.fc1_wt, args.fc1_b, args.fc2_w,
-                         args.fc2_b, raw, out, n, kP, args.hidden, stream);
+                         args.fc2_b, raw, out, nboards, kP, args.hidden,
+                         stream);
 }
 
 }  // namespace
 
 int fused_blocks_per_cu(int which) {
-    // The BF16 and FP16 instantiations share shapes and resources.
+    // Every instantiation of one net shares its resource footprint up to
+    // the ragged tile's few extra rows.
     int blocks = 0;
     hipError_t e =
         which == 0
             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, Kernels<__bf16>::policy, kPolicyThreads, 0)
+                  &blocks, Kernels<__bf16, true>::policy, kPolicyThreads, 0)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, Kernels<__bf16>::value, kValueThreads, 0);
+                  &blocks, Kernels<__bf16, true>::value, kValueThreads, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
+}
+
+int ragged_slot_capacity(int side) {
+    if (side < 5 || side > 19) return 0;
+    int k = 1;
+    while (k < 4 && (k + 1) * side * side <= 384 &&
+           (1 + (k + 1) * (side + 1)) * (side + 2) <= kTilePixels)
+        k++;
+    return k;
 }
 
 hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
                          const FusedArgs &args, float *raw, float *out, int n,
                          float temperature, int grid, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    if (grid <= 0 || grid > n) grid = n;
+    const bool ragged = args.slots != nullptr;
+    const int nslots = ragged ? args.nslots : n;
     if (type == FusedType::kBF16)
-        return launch<__bf16>(which, planes, args, raw, out, n, temperature,
-                              grid, stream);
-    return launch<_Float16>(which, planes, args, raw, out, n, temperature,
-                            grid, stream);
+        return ragged ? launch<__bf16, true>(which, planes, args, raw, out,
+                                             nslots, n, temperature, grid,
+                                             stream)
+                      : launch<__bf16, false>(which, planes, args, raw, out,
+                                              nslots, n, temperature, grid,
+                                              stream);
+    return ragged ? launch<_Float16, true>(which, planes, args, raw, out,
+                                           nslots, n, temperature, grid, stream)
+                  : launch<_Float16, false>(which, planes, args, raw, out,
+                                            nslots, n, temperature, grid,
+                                            stream);
 }
 
 }  // namespace lz

@@ -180,6 +180,76 @@ int lz_set_softmax_temp(lz_ctx *ctx, float temperature) {
     return LZ_OK;
 }
 
+int lz_forward_ragged_device(lz_ctx *ctx, int net, int precision, int n,
+                             const int32_t *sides, const uint8_t *d_planes,
+                             float *d_raw, float *d_out, void *stream) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (!ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
+    if (n < 0 || (n > 0 && (!sides || !d_planes)))
+        return ctx->fail(LZ_ERR_ARG, "bad ragged batch");
+    if (n == 0) return LZ_OK;
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return ctx->fail(LZ_ERR_ARG, "bad net id %d", net);
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    std::string msg;
+    int rc = lz::build_ragged_layout(ctx->ragged, n, sides, s, &msg);
+    if (rc == LZ_OK) rc = lz::ensure_workspace(ctx->ws, ctx->net, n, &msg);
+    if (rc == LZ_OK)
+        rc = lz::run_net(ctx->net[net], net, precision, n, d_planes, d_raw,
+                         d_out, ctx->temperature, ctx->ws, s, &msg, nullptr,
+                         &ctx->ragged);
+    if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    return LZ_OK;
+}
+
+int lz_forward_ragged(lz_ctx *ctx, int net, int precision, int n,
+                      const int32_t *sides, const uint8_t *planes, float *raw,
+                      float *out) {
+    if (!ctx) return LZ_ERR_ARG;
+    if (n < 0 || (n > 0 && (!sides || !planes))) return LZ_ERR_ARG;
+    if (n == 0) return LZ_OK;
+    size_t pbytes = 0, pfloats = 0;
+    for (int i = 0; i < n; i++) {
+        if (sides[i] < 5 || sides[i] > 19) return LZ_ERR_ARG;
+        pbytes += (size_t)32 * sides[i] * sides[i];
+        pfloats += (size_t)sides[i] * sides[i];
+    }
+    const size_t outs = net == LZ_NET_POLICY ? pfloats : (size_t)n;
+    uint8_t *d_planes = nullptr;
+    float *d_raw = nullptr, *d_out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&d_planes), pbytes);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&d_raw), sizeof(float) * outs);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&d_out), sizeof(float) * outs);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_planes, planes, pbytes, hipMemcpyHostToDevice,
+                           ctx->stream);
+    int rc = e == hipSuccess ? LZ_OK : LZ_ERR_HIP;
+    if (rc == LZ_OK)
+        rc = lz_forward_ragged_device(ctx, net, precision, n, sides, d_planes,
+                                      d_raw, d_out, nullptr);
+    if (rc == LZ_OK && raw)
+        e = hipMemcpyAsync(raw, d_raw, sizeof(float) * outs,
+                           hipMemcpyDeviceToHost, ctx->stream);
+    if (rc == LZ_OK && e == hipSuccess && out)
+        e = hipMemcpyAsync(out, d_out, sizeof(float) * outs,
+                           hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (rc == LZ_OK && e != hipSuccess) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        rc = ctx->fail(LZ_ERR_HIP, "ragged forward: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(d_planes);
+    (void)hipFree(d_raw);
+    (void)hipFree(d_out);
+    return rc;
+}
+
 int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
                      const uint8_t *d_planes, float *d_raw, float *d_out,
                      void *stream, float *launch_ms, int max_launches,
@@ -279,6 +349,7 @@ void lz_destroy(lz_ctx *ctx) {
     lz::free_net(ctx->net[0]);
     lz::free_net(ctx->net[1]);
     lz::free_workspace(ctx->ws);
+    lz::free_ragged_layout(ctx->ragged);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

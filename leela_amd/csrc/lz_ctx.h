@@ -23,6 +23,7 @@ struct lz_ctx {
     std::mutex mu;  // guards the nets, workspace and err
     lz::Net net[2];
     lz::Workspace ws;
+    lz::RaggedLayout ragged;  // layout of the last ragged batch
     float temperature = 0.75f;
     bool finalized = false;
     lz::LeafQueue *queue = nullptr;  // owned; see lz::destroy_queue

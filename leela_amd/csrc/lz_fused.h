@@ -8,6 +8,19 @@ namespace lz {
 
 constexpr int kMaxFusedConvs = 16;
 
+// One workgroup pass of a ragged batch: up to four boards of one side,
+// stacked in the LDS tile (board ids index in_off / out_off / h0 rows).
+struct FusedSlot {
+    int side;
+    int count;
+    int board[4];
+};
+
+// Boards of `side` one slot holds (0 for unsupported sides): 1 for 19x19,
+// 2 for 13x13, 4 for 9x9; limited by the 384-pixel column tiling and the
+// 456-pixel LDS tile.
+int ragged_slot_capacity(int side);
+
 // Device pointers of one net in the fused kernels' layouts. w[l] holds conv
 // l packed (in the operand type) in MFMA fragment order: [k-step][m-tile][lane 0..63][8 values],
 // k-steps tap-major with 32 input channels each (see pack_conv_fragments).
@@ -19,6 +32,11 @@ struct FusedArgs {
     const float *fc2_w = nullptr;
     const float *fc2_b = nullptr;
     float *h0 = nullptr;            // value head plane [n][361], per call
+    // Ragged batches (per call; slots == nullptr for n 19x19 boards):
+    const FusedSlot *slots = nullptr;
+    int nslots = 0;
+    const int *in_off = nullptr;    // board -> byte offset of its planes
+    const int *out_off = nullptr;   // board -> offset of its policy outputs
     int hidden = 0;
     int nconv = 0;
     int c = 0;   // trunk width

@@ -311,7 +311,12 @@ static int run_fp32(const Net &net, int which, int n, const uint8_t *planes,
 int run_net(const Net &net, int which, int precision, int n,
             const uint8_t *d_planes, float *d_raw, float *d_out,
             float temperature, Workspace &ws, hipStream_t stream,
-            std::string *msg, LaunchTimer *timer) {
+            std::string *msg, LaunchTimer *timer,
+            const RaggedLayout *ragged) {
+    if (ragged && precision != LZ_PREC_BF16 && precision != LZ_PREC_FP16) {
+        *msg = "ragged batches run on the 16-bit fused kernels only";
+        return LZ_ERR_UNSUPPORTED;
+    }
     if (precision == LZ_PREC_FP32)
         return run_fp32(net, which, n, d_planes, d_raw, d_out, temperature,
                         ws, stream, msg, timer);
@@ -325,6 +330,12 @@ int run_net(const Net &net, int which, int precision, int n,
         hipError_t e = timer ? timer->mark(stream) : hipSuccess;
         FusedArgs args = bf ? net.fused : net.fused16;
         args.h0 = ws.h0;
+        if (ragged) {
+            args.slots = ragged->d_slots;
+            args.nslots = ragged->nslots;
+            args.in_off = ragged->d_in_off;
+            args.out_off = ragged->d_out_off;
+        }
         if (e == hipSuccess)
             e = fused_forward(which, bf ? FusedType::kBF16 : FusedType::kFP16,
                               d_planes, args, d_raw, d_out, n, temperature,
@@ -354,6 +365,83 @@ void normal_fill(uint32_t seed, size_t n, float stddev, float *out) {
         out[i++] = (float)(stddev * r * std::cos(two_pi * u2));
         if (i < n) out[i++] = (float)(stddev * r * std::sin(two_pi * u2));
     }
+}
+
+}  // namespace lz
+
+namespace lz {
+
+int build_ragged_layout(RaggedLayout &lay, int n, const int32_t *sides,
+                        hipStream_t stream, std::string *msg) {
+    std::vector<int> in_off(n), out_off(n);
+    std::vector<FusedSlot> slots;
+    size_t pb = 0, pf = 0;
+    for (int i = 0; i < n; i++) {
+        if (ragged_slot_capacity(sides[i]) == 0) {
+            *msg = "board side " + std::to_string(sides[i]) +
+                   " outside 5..19";
+            return LZ_ERR_ARG;
+        }
+        in_off[i] = (int)pb;
+        out_off[i] = (int)pf;
+        pb += (size_t)32 * sides[i] * sides[i];
+        pf += (size_t)sides[i] * sides[i];
+    }
+    // Group by side in first-appearance order; fill slots to capacity.
+    std::vector<int> order;
+    std::vector<char> done(n, 0);
+    for (int i = 0; i < n; i++) {
+        if (done[i]) continue;
+        const int s = sides[i], cap = ragged_slot_capacity(s);
+        FusedSlot cur{s, 0, {0, 0, 0, 0}};
+        for (int j = i; j < n; j++) {
+            if (done[j] || sides[j] != s) continue;
+            done[j] = 1;
+            cur.board[cur.count++] = j;
+            if (cur.count == cap) {
+                slots.push_back(cur);
+                cur = FusedSlot{s, 0, {0, 0, 0, 0}};
+            }
+        }
+        if (cur.count) slots.push_back(cur);
+    }
+    hipError_t e = hipSuccess;
+    if (n > lay.cap_boards || (int)slots.size() > lay.cap_slots) {
+        free_ragged_layout(lay);
+        e = dalloc(&lay.d_slots, std::max<size_t>(slots.size(), 1));
+        if (e == hipSuccess) e = dalloc(&lay.d_in_off, (size_t)std::max(n, 1));
+        if (e == hipSuccess) e = dalloc(&lay.d_out_off, (size_t)std::max(n, 1));
+        lay.cap_boards = n;
+        lay.cap_slots = (int)slots.size();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(lay.d_slots, slots.data(),
+                           sizeof(FusedSlot) * slots.size(),
+                           hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(lay.d_in_off, in_off.data(), sizeof(int) * n,
+                           hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(lay.d_out_off, out_off.data(), sizeof(int) * n,
+                           hipMemcpyHostToDevice, stream);
+    // The host vectors go out of scope: finish the copies first.
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        *msg = std::string("ragged layout upload: ") + hipGetErrorString(e);
+        return LZ_ERR_HIP;
+    }
+    lay.n = n;
+    lay.nslots = (int)slots.size();
+    lay.plane_bytes = pb;
+    lay.policy_floats = pf;
+    return LZ_OK;
+}
+
+void free_ragged_layout(RaggedLayout &lay) {
+    dfree(lay.d_slots);
+    dfree(lay.d_in_off);
+    dfree(lay.d_out_off);
+    lay = RaggedLayout();
 }
 
 }  // namespace lz

@@ -54,6 +54,23 @@ struct Workspace {
     float *h0 = nullptr;     // value head plane [cap][361]
 };
 
+// Device copy of a ragged batch's layout: slots and per-board offsets.
+struct RaggedLayout {
+    int n = 0, nslots = 0;
+    size_t plane_bytes = 0;   // total planes bytes of the batch
+    size_t policy_floats = 0; // total policy outputs of the batch
+    FusedSlot *d_slots = nullptr;
+    int *d_in_off = nullptr;
+    int *d_out_off = nullptr;
+    int cap_boards = 0, cap_slots = 0;
+};
+
+// Packs boards of sides[0..n) into slots (grouped by side, stable) and
+// uploads the layout on `stream`. Sides must be 5..19.
+int build_ragged_layout(RaggedLayout &lay, int n, const int32_t *sides,
+                        hipStream_t stream, std::string *msg);
+void free_ragged_layout(RaggedLayout &lay);
+
 int validate_net(Net &net, int which, std::string *msg);
 int upload_net(Net &net, int which, std::string *msg);
 void free_net(Net &net);
@@ -71,7 +88,8 @@ struct LaunchTimer {
 int run_net(const Net &net, int which, int precision, int n,
             const uint8_t *d_planes, float *d_raw, float *d_out,
             float temperature, Workspace &ws, hipStream_t stream,
-            std::string *msg, LaunchTimer *timer = nullptr);
+            std::string *msg, LaunchTimer *timer = nullptr,
+            const RaggedLayout *ragged = nullptr);
 
 // Host packing of a conv into MFMA fragment order (16-bit values, round to
 // nearest even): [k-step][m-tile][lane][8], k-step s = tap * (cin/32) +

@@ -306,6 +306,28 @@ static int max_col(int nlayers, const orc_layer *L, int S) {
     return m;
 }
 
+/* The reference nets are 19x19 only (Network.cpp:541-543). For a smaller
+ * board the value head's 361-input inner product sees the S x S plane at
+ * index y*19+x with zeros elsewhere (the library's definition); `a` holds
+ * the S*S plane and is rewritten in place as the 361-vector. */
+static void embed_plane_f32(float *a, int S) {
+    float tmp[361];
+    for (int i = 0; i < 361; i++) {
+        const int y = i / 19, x = i % 19;
+        tmp[i] = (y < S && x < S) ? a[y * S + x] : 0.0f;
+    }
+    memcpy(a, tmp, sizeof tmp);
+}
+
+static void embed_plane_f64(double *a, int S) {
+    double tmp[361];
+    for (int i = 0; i < 361; i++) {
+        const int y = i / 19, x = i % 19;
+        tmp[i] = (y < S && x < S) ? a[y * S + x] : 0.0;
+    }
+    memcpy(a, tmp, sizeof tmp);
+}
+
 static void fwd_one_f32(const fwd_job *j, int pos, float *a, float *b,
                         float *col) {
     const int S = j->S, P = S * S, cin0 = j->L[0].cin;
@@ -319,6 +341,7 @@ static void fwd_one_f32(const fwd_job *j, int pos, float *a, float *b,
                              col);
             len = L->cout * P;
         } else {
+            if (L->cin == 361 && len == S * S && S < 19) embed_plane_f32(a, S);
             orc_innerproduct_f32(L->cin, L->cout, a, L->w, L->b, b);
             len = L->cout;
         }
@@ -372,12 +395,16 @@ static void fwd_one_f64(const fwd_job *j, int pos, double *a, double *b) {
     const int S = j->S, P = S * S, cin0 = j->L[0].cin;
     const uint8_t *pl = j->planes + (size_t)pos * cin0 * P;
     for (int i = 0; i < cin0 * P; i++) a[i] = (double)pl[i];
+    int len = cin0 * P;
     for (int l = 0; l < j->nlayers; l++) {
         const orc_layer *L = &j->L[l];
         if (L->kind == ORC_CONV) {
             orc_conv_direct_f64(L->filter, L->cin, L->cout, S, a, L->w, L->b,
                                 b, 1);
+            len = L->cout * P;
         } else {
+            if (L->cin == 361 && len == P && S < 19) embed_plane_f64(a, S);
+            len = L->cout;
             for (int o = 0; o < L->cout; o++) {
                 double acc = (double)L->b[o];
                 for (int i = 0; i < L->cin; i++)
@@ -450,7 +477,11 @@ static int check_layers(int nlayers, const orc_layer *L, int S, int head) {
                 return -3;
             width = L[i].cout * S * S;
         } else {
-            if (L[i].cin != width) return -4;
+            /* A 1-channel S x S plane may feed a 361-input layer when S < 19
+             * (build-defined embedding, see embed_plane). */
+            const int embedded = spatial && width == S * S && S < 19 &&
+                                 L[i].cin == 361;
+            if (L[i].cin != width && !embedded) return -4;
             spatial = 0;
             width = L[i].cout;
         }
