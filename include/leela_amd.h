@@ -115,6 +115,19 @@ int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
                      void *stream, float *launch_ms, int max_launches,
                      int *n_launches);
 
+/* ---- Autotuner ----------------------------------------------------------
+ * Times every launch shape of the fused kernels (wave grid 1x4 or 2x4;
+ * persistent grid of CUs x occupancy x {1, 2} or one workgroup per slot)
+ * for batches of n on the device and keeps the fastest per net for n's
+ * bucket (<=64, <=256, <=1024, more). Replaces the reference's fixed
+ * OpenCL work sizes (cfg_rowtiles, OpenCL.cpp:605-619). report (optional)
+ * receives one line per candidate. Tuning is per context; save/load keep
+ * it in a text file. */
+int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
+                size_t report_len);
+int lz_autotune_save(lz_ctx *ctx, const char *path);
+int lz_autotune_load(lz_ctx *ctx, const char *path);
+
 /* ---- Batched leaf queue -------------------------------------------------
  * Replaces the per-thread asynchronous path: Network::async_scored_moves
  * -> OpenCL_Network::forward with forward_cb (Network.cpp:471-587,

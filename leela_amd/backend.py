@@ -74,6 +74,9 @@ def load_library():
     L.lz_forward_ragged_device.argtypes = [vp, C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_int32), vp, vp, vp,
                                            vp]
+    L.lz_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+    L.lz_autotune_save.argtypes = [vp, C.c_char_p]
+    L.lz_autotune_load.argtypes = [vp, C.c_char_p]
     L.lz_queue_start.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.lz_submit.argtypes = [vp, C.c_int, C.POINTER(C.c_uint8), DONE_FN, vp]
     L.lz_flush.argtypes = [vp]
@@ -216,6 +219,22 @@ class Backend:
         self._check(self._L.lz_forward_ragged_device(
             self._ctx, net, precision, len(sides), _ptr(sides, C.c_int32),
             d_planes, d_raw, d_out, stream), "lz_forward_ragged_device")
+
+    def autotune(self, precision: int, n: int) -> str:
+        """Times the fused kernels' launch shapes for batches of n and keeps
+        the fastest; returns the timing report."""
+        buf = C.create_string_buffer(8192)
+        self._check(self._L.lz_autotune(self._ctx, precision, n, buf,
+                                        len(buf)), "lz_autotune")
+        return buf.value.decode()
+
+    def autotune_save(self, path: str):
+        self._check(self._L.lz_autotune_save(self._ctx, path.encode()),
+                    "lz_autotune_save")
+
+    def autotune_load(self, path: str):
+        self._check(self._L.lz_autotune_load(self._ctx, path.encode()),
+                    "lz_autotune_load")
 
     # ---- batched leaf queue (lz_queue_start / lz_submit / lz_join) ----
 

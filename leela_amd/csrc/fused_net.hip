@@ -326,35 +326,37 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
 // Policy: one 8-wave workgroup per CU (121-124 KB LDS tile), 2 waves per
 // SIMD. Value: 4-wave workgroups, two per CU (65 KB tile each), 2 waves per
 // SIMD.
-constexpr int kPolicyWM = 2, kValueWM = 1;
-constexpr int kPolicyThreads = kPolicyWM * kWN * 64;
-constexpr int kValueThreads = kValueWM * kWN * 64;
-
-template <typename T, bool RAGGED>
-struct Kernels {
-    static constexpr auto policy =
-        fused_net<T, 128, 96, 11, false, RAGGED, kPolicyWM, 2>;
-    static constexpr auto value =
-        fused_net<T, 64, 64, 10, true, RAGGED, kValueWM, 2>;
+// Wave grids the autotuner chooses from. Policy: 2 x 4 (8 waves, 96
+// accumulators per lane, 2 waves/SIMD; default) or 1 x 4 (4 waves, 192
+// accumulators, 1 wave/SIMD). Value: 1 x 4 (4 waves, two workgroups per CU;
+// default) or 2 x 4 (8 waves).
+template <typename T, bool RAGGED, int WM>
+struct PolicyKernel {
+    static constexpr auto fn =
+        fused_net<T, 128, 96, 11, false, RAGGED, WM, WM == 2 ? 2 : 1>;
+};
+template <typename T, bool RAGGED, int WM>
+struct ValueKernel {
+    static constexpr auto fn = fused_net<T, 64, 64, 10, true, RAGGED, WM, 2>;
 };
 
-template <typename T, bool RAGGED>
-hipError_t launch(int which, const uint8_t *planes, const FusedArgs &args,
-                  float *raw, float *out, int nslots, int nboards,
-                  float temperature, int grid, hipStream_t stream) {
-    if (grid <= 0 || grid > nslots) grid = nslots;
+template <typename T, bool RAGGED, int WM>
+hipError_t launch_wm(int which, const uint8_t *planes, const FusedArgs &args,
+                     float *raw, float *out, int nslots, int nboards,
+                     float temperature, int grid, hipStream_t stream) {
+    const int threads = WM * kWN * 64;
     if (which == 0) {
         if (args.nconv != 13 || args.c != 128 || args.m1 != 96)
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL((Kernels<T, RAGGED>::policy), dim3(grid),
-                           dim3(kPolicyThreads), 0, stream, planes, args, raw,
-                           out, nslots, temperature);
+        hipLaunchKernelGGL((PolicyKernel<T, RAGGED, WM>::fn), dim3(grid),
+                           dim3(threads), 0, stream, planes, args, raw, out,
+                           nslots, temperature);
         return hipGetLastError();
     }
     if (args.nconv != 12 || args.c != 64 || args.m1 != 64 || !args.h0)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL((Kernels<T, RAGGED>::value), dim3(grid),
-                       dim3(kValueThreads), 0, stream, planes, args, raw, out,
+    hipLaunchKernelGGL((ValueKernel<T, RAGGED, WM>::fn), dim3(grid),
+                       dim3(threads), 0, stream, planes, args, raw, out,
                        nslots, temperature);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -363,18 +365,39 @@ hipError_t launch(int which, const uint8_t *planes, const FusedArgs &args,
                          stream);
 }
 
+template <typename T, bool RAGGED>
+hipError_t launch(int which, int wm, const uint8_t *planes,
+                  const FusedArgs &args, float *raw, float *out, int nslots,
+                  int nboards, float temperature, int grid,
+                  hipStream_t stream) {
+    if (grid <= 0 || grid > nslots) grid = nslots;
+    return wm == 1 ? launch_wm<T, RAGGED, 1>(which, planes, args, raw, out,
+                                             nslots, nboards, temperature,
+                                             grid, stream)
+                   : launch_wm<T, RAGGED, 2>(which, planes, args, raw, out,
+                                             nslots, nboards, temperature,
+                                             grid, stream);
+}
+
 }  // namespace
 
-int fused_blocks_per_cu(int which) {
-    // Every instantiation of one net shares its resource footprint up to
-    // the ragged tile's few extra rows.
+int fused_default_wm(int which) { return which == 0 ? 2 : 1; }
+
+int fused_blocks_per_cu(int which, int wm) {
+    // The ragged instantiation has the larger LDS tile of the two.
     int blocks = 0;
-    hipError_t e =
-        which == 0
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, Kernels<__bf16, true>::policy, kPolicyThreads, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                  &blocks, Kernels<__bf16, true>::value, kValueThreads, 0);
+    hipError_t e;
+    const int threads = wm * kWN * 64;
+    if (which == 0)
+        e = wm == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &blocks, PolicyKernel<__bf16, true, 1>::fn, threads, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &blocks, PolicyKernel<__bf16, true, 2>::fn, threads, 0);
+    else
+        e = wm == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &blocks, ValueKernel<__bf16, true, 1>::fn, threads, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &blocks, ValueKernel<__bf16, true, 2>::fn, threads, 0);
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
@@ -389,20 +412,23 @@ int ragged_slot_capacity(int side) {
 
 hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
                          const FusedArgs &args, float *raw, float *out, int n,
-                         float temperature, int grid, hipStream_t stream) {
+                         float temperature, const FusedLaunch &cfg,
+                         hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const bool ragged = args.slots != nullptr;
     const int nslots = ragged ? args.nslots : n;
+    const int wm = cfg.wm == 1 || cfg.wm == 2 ? cfg.wm : fused_default_wm(which);
+    const int grid = cfg.grid;
     if (type == FusedType::kBF16)
-        return ragged ? launch<__bf16, true>(which, planes, args, raw, out,
+        return ragged ? launch<__bf16, true>(which, wm, planes, args, raw, out,
                                              nslots, n, temperature, grid,
                                              stream)
-                      : launch<__bf16, false>(which, planes, args, raw, out,
-                                              nslots, n, temperature, grid,
-                                              stream);
-    return ragged ? launch<_Float16, true>(which, planes, args, raw, out,
+                      : launch<__bf16, false>(which, wm, planes, args, raw,
+                                              out, nslots, n, temperature,
+                                              grid, stream);
+    return ragged ? launch<_Float16, true>(which, wm, planes, args, raw, out,
                                            nslots, n, temperature, grid, stream)
-                  : launch<_Float16, false>(which, planes, args, raw, out,
+                  : launch<_Float16, false>(which, wm, planes, args, raw, out,
                                             nslots, n, temperature, grid,
                                             stream);
 }

@@ -50,12 +50,23 @@ enum class FusedType { kBF16 = 0, kFP16 = 1 };
 // which: 0 policy (13 convs: 5x5 32->96, 3x3 96->128, 10x 128->128,
 // 128->1), 1 value (12 convs: 5x5 32->64, 10x 64->64, 64->1, + 2 FC).
 // grid <= 0 launches one workgroup per board.
-// Resident workgroups per CU of the fused kernel of net `which` (LDS and
-// register limited); the persistent grid is CUs times this.
-int fused_blocks_per_cu(int which);
+// Launch shape of a fused forward: wm waves along output channels (1 or 2;
+// 0 = the net's default) and the persistent grid (workgroups; <= 0 or more
+// than the slots = one workgroup per slot).
+struct FusedLaunch {
+    int wm = 0;
+    int grid = 0;
+};
+
+int fused_default_wm(int which);
+
+// Resident workgroups per CU of the fused kernel of net `which` with wave
+// grid wm x 4 (LDS and register limited).
+int fused_blocks_per_cu(int which, int wm);
 
 hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
                          const FusedArgs &args, float *raw, float *out, int n,
-                         float temperature, int grid, hipStream_t stream);
+                         float temperature, const FusedLaunch &cfg,
+                         hipStream_t stream);
 
 }  // namespace lz

@@ -161,10 +161,25 @@ int upload_net(Net &net, int which, std::string *msg) {
     net.fused16 = A;
     for (int k = 0; k < nconv; k++) net.fused16.w[k] = net.layers[k].d_wf16;
     int dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    net.fused_grid = cus * fused_blocks_per_cu(which);
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                dev);
+    net.cus = cus > 0 ? cus : 1;
+    net.occ[1] = fused_blocks_per_cu(which, 1);
+    net.occ[2] = fused_blocks_per_cu(which, 2);
     return LZ_OK;
+}
+
+int tune_bucket(int n) {
+    return n <= 64 ? 0 : n <= 256 ? 1 : n <= 1024 ? 2 : 3;
+}
+
+FusedLaunch fused_launch_for(const Net &net, int which, int nslots) {
+    const Net::Tune &t = net.tune[tune_bucket(nslots)];
+    FusedLaunch cfg;
+    cfg.wm = t.wm == 1 || t.wm == 2 ? t.wm : fused_default_wm(which);
+    cfg.grid = t.grid_mult <= 0 ? 0 : net.cus * net.occ[cfg.wm] * t.grid_mult;
+    return cfg;
 }
 
 uint16_t float_to_bf16_rne(float f) {
@@ -336,10 +351,12 @@ int run_net(const Net &net, int which, int precision, int n,
             args.in_off = ragged->d_in_off;
             args.out_off = ragged->d_out_off;
         }
+        const FusedLaunch cfg =
+            fused_launch_for(net, which, ragged ? ragged->nslots : n);
         if (e == hipSuccess)
             e = fused_forward(which, bf ? FusedType::kBF16 : FusedType::kFP16,
                               d_planes, args, d_raw, d_out, n, temperature,
-                              net.fused_grid, stream);
+                              cfg, stream);
         if (e == hipSuccess && timer) e = timer->mark(stream);
         if (e != hipSuccess) {
             *msg = std::string("16-bit forward launch: ") +

@@ -42,8 +42,20 @@ struct Net {
     bool fused_ok = false;  // shapes the fused kernels implement
     FusedArgs fused;        // device pointers for the BF16 fused kernel
     FusedArgs fused16;      // device pointers for the FP16 fused kernel
-    int fused_grid = 0;     // persistent workgroups (CUs x occupancy)
+    int cus = 0;            // compute units of the device
+    int occ[3] = {0, 0, 0}; // resident workgroups per CU for wm = 1, 2
+    // Launch shape per batch bucket (<=64, <=256, <=1024, more), set by
+    // the autotuner: wm 0 = the net's default wave grid; grid_mult 0 = one
+    // workgroup per slot, else CUs x occupancy x grid_mult (persistent).
+    struct Tune {
+        int wm = 0;
+        int grid_mult = 1;
+        float ms = 0.0f;  // measured forward time of the choice
+    } tune[4];
 };
+
+int tune_bucket(int n);
+FusedLaunch fused_launch_for(const Net &net, int which, int nslots);
 
 struct Workspace {
     int cap = 0;             // positions the buffers hold
