@@ -58,6 +58,13 @@ def main() -> int:
                     choices=["auto", "fp32", "bf16", "fp16"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-only", choices=["fp32", "bf16", "fp16"],
+                    help="profiling workload: only `steps` policy forwards "
+                         "of this form (plus --copy-mib), no JSON contract")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="with --kernel-only: a fill and an elementwise copy "
+                         "of this many MiB, the known-size reference for PMC "
+                         "byte counts")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: leaf positions are independent and "
@@ -105,6 +112,25 @@ def main() -> int:
         win = d_vout.cpu().numpy().copy()
         return {"dt": dt, "steps": steps, "pos_s": B * steps / dt,
                 "logits": logits, "win": win}
+
+    if args.kernel_only:
+        p = leela_amd.PRECISIONS[args.kernel_only]
+        for _ in range(args.warmup + args.steps):
+            be.forward_device(leela_amd.NET_POLICY, p, B, d_planes.data_ptr(),
+                              d_praw.data_ptr(), d_pout.data_ptr(), stream)
+        if args.copy_mib:
+            # Fill (writes N MiB), then y = x + 1 (reads N MiB, writes N
+            # MiB): the known-size kernels that calibrate the byte counters.
+            src = torch.ones(args.copy_mib << 18, dtype=torch.float32,
+                             device=dev)
+            dst = torch.empty_like(src)
+            torch.add(src, 1.0, out=dst)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"kernel_only": args.kernel_only, "batch": B,
+                          "policy_forwards": args.warmup + args.steps,
+                          "copy_bytes": args.copy_mib << 20}))
+        be.close()
+        return 0
 
     forms = ["fp32", "bf16", "fp16"]
     res = {}

@@ -257,6 +257,29 @@ typedef void (*lz_moves_fn)(void *user, int count, const float *probs,
 int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
                                   int rotation, lz_moves_fn done, void *user);
 
+/* ---- Monte Carlo playouts (teacher and fallback of the surrogate) -----
+ * Playout::mc_owner (Playout.cpp:174-217): the mean outcome of n playouts
+ * from pos for the side to move (win 1, tie 0.5). Each playout follows
+ * Playout::run (Playout.cpp:51-113: stop at four passes or move number
+ * 2*side^2, score by FastBoard::final_mc_score) with a light policy —
+ * uniform over empty points that are not the ko square, an own eye or
+ * suicide — in place of the reference's pattern policy. Host only. */
+int lz_playout_winrate(const lz_position *pos, int n, uint32_t seed,
+                       float *winrate);
+
+/* FastBoard::final_mc_score (FastBoard.cpp:420-443) with the position's
+ * komi: black stones plus empty points whose four neighbours are black or
+ * edge, minus the same for white, minus komi. */
+int lz_position_score(const lz_position *pos, float *score);
+
+/* Training data of the surrogate: generator positions from seeds
+ * seed0..seed0+n-1 (as lz_make_batch), their value planes (if planes is not
+ * NULL) and their n-playout win rates for the side to move, playouts of
+ * position i seeded playout_seed + i. */
+int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
+                         int playouts, uint32_t playout_seed, int threads,
+                         uint8_t *planes, float *winrates);
+
 /* Device name, as OpenCL::get_device_name (OpenCL.h:121). */
 const char *lz_device_name(lz_ctx *ctx);
 

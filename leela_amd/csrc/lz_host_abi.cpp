@@ -11,6 +11,7 @@
 #include "input_planes.h"
 #include "leela_amd.h"
 #include "lz_position.h"
+#include "playout.h"
 
 namespace {
 
@@ -134,6 +135,44 @@ int lz_make_batch(int n, int side, uint32_t seed0, int max_moves, int net,
                 lz::gather_features_policy(st, planes + per * i);
             else
                 lz::gather_features_value(st, planes + per * i);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    return LZ_OK;
+}
+
+int lz_position_score(const lz_position *pos, float *score) {
+    if (!pos || !score) return LZ_ERR_ARG;
+    *score = lz::go::final_mc_score(pos->st.board, pos->st.komi);
+    return LZ_OK;
+}
+
+int lz_playout_winrate(const lz_position *pos, int n, uint32_t seed,
+                       float *winrate) {
+    if (!pos || !winrate || n < 0) return LZ_ERR_ARG;
+    *winrate = lz::go::mc_winrate(pos->st, n, seed);
+    return LZ_OK;
+}
+
+int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
+                         int playouts, uint32_t playout_seed, int threads,
+                         uint8_t *planes, float *winrates) {
+    if (n < 0 || side < 5 || side > lz::go::kMaxSide || max_moves < 0 ||
+        playouts < 1 || !winrates)
+        return LZ_ERR_ARG;
+    const size_t per = (size_t)lz::kPlanes * side * side;
+    if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+    threads = std::max(1, std::min(threads, std::max(n, 1)));
+    auto work = [&](int t) {
+        lz::go::GameState st;
+        for (int i = t; i < n; i += threads) {
+            lz::go::random_position(st, side, seed0 + (uint32_t)i, max_moves);
+            if (planes) lz::gather_features_value(st, planes + per * i);
+            winrates[i] =
+                lz::go::mc_winrate(st, playouts, playout_seed + (uint32_t)i);
         }
     };
     std::vector<std::thread> pool;

@@ -1,0 +1,33 @@
+// playout.h — Monte Carlo playouts: the teacher (and fallback) of the
+// learned leaf-value surrogate.
+#pragma once
+
+#include <cstdint>
+
+#include "go_board.h"
+#include "lz_random.h"
+
+namespace lz {
+namespace go {
+
+// FastBoard::final_mc_score (FastBoard.cpp:420-443): black stones plus
+// empty points whose four neighbours are all black (or edge), minus the
+// same for white, minus komi.
+float final_mc_score(const Board &b, float komi);
+
+// One playout from `st` (played in place): Playout::run's loop
+// (Playout.cpp:51-113) with mc_owner's settings — stop at 4 passes in a
+// row or at move number 2 * side^2 — where each move is uniform over the
+// empty points that are not the ko square, not an own eye and not suicide
+// (pass when there is none). The reference draws moves from its pattern
+// policy (FastState::play_random_move, FastState.cpp:168-255); this light
+// policy is the build's teacher. Returns final_mc_score with the state's
+// komi.
+float playout(GameState &st, Rng &rng);
+
+// Playout::mc_owner (Playout.cpp:174-217): mean outcome of n playouts for
+// the side to move (win 1, tie 0.5, loss 0), streams from `seed`.
+float mc_winrate(const GameState &st, int n, uint32_t seed);
+
+}  // namespace go
+}  // namespace lz

@@ -41,6 +41,12 @@ def _lib():
         L.lz_board_queries.argtypes = [vp, i32p]
         L.lz_make_batch.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_int,
                                     C.c_int, C.c_int, u8p]
+        f32p = C.POINTER(C.c_float)
+        L.lz_position_score.argtypes = [vp, f32p]
+        L.lz_playout_winrate.argtypes = [vp, C.c_int, C.c_uint32, f32p]
+        L.lz_make_value_labels.argtypes = [C.c_int, C.c_int, C.c_uint32,
+                                           C.c_int, C.c_int, C.c_uint32,
+                                           C.c_int, u8p, f32p]
         L._positions_bound = True
     return L
 
@@ -106,6 +112,20 @@ class Position:
             C.POINTER(C.c_int32))), "queries")
         return a
 
+    def score(self) -> float:
+        """FastBoard::final_mc_score with the position's komi."""
+        f = C.c_float()
+        _chk(self._L.lz_position_score(self._p, C.byref(f)), "score")
+        return f.value
+
+    def playout_winrate(self, n: int, seed: int) -> float:
+        """Playout::mc_owner: mean outcome of n light playouts for the
+        side to move (tie 0.5)."""
+        f = C.c_float()
+        _chk(self._L.lz_playout_winrate(self._p, n, seed, C.byref(f)),
+             "playout")
+        return f.value
+
 
 def make_batch(n: int, seed0: int = 1, side: int = 19,
                max_moves: int = MAX_MOVES_19, net: int = NET_POLICY,
@@ -116,6 +136,22 @@ def make_batch(n: int, seed0: int = 1, side: int = 19,
                               out.ctypes.data_as(C.POINTER(C.c_uint8))),
          "make_batch")
     return out
+
+
+def value_labels(n: int, seed0: int, playouts: int, playout_seed: int,
+                 side: int = 19, max_moves: int = MAX_MOVES_19,
+                 threads: int = 0, planes: bool = True):
+    """Surrogate training data: value planes [n][32][side*side] (or None)
+    of generator positions from seeds seed0.., and their `playouts`-playout
+    win rates for the side to move, playouts of position i seeded
+    playout_seed + i."""
+    pl = np.zeros((n, 32, side * side), np.uint8) if planes else None
+    w = np.zeros(n, np.float32)
+    _chk(_lib().lz_make_value_labels(
+        n, side, seed0, max_moves, playouts, playout_seed, threads,
+        pl.ctypes.data_as(C.POINTER(C.c_uint8)) if planes else None,
+        w.ctypes.data_as(C.POINTER(C.c_float))), "value_labels")
+    return pl, w
 
 
 def batch_planes(n: int, net: int = 0, seed0: int = 1) -> np.ndarray:

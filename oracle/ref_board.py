@@ -49,6 +49,8 @@ def lib():
                                   C.POINTER(C.c_uint32)]
         L.ref_rng_u32.restype = None
         L.ref_rng_flt.argtypes = [C.c_uint32, C.c_int, C.POINTER(C.c_float)]
+        L.ref_final_score.argtypes = [C.c_int, C.c_float, C.c_int, i32p]
+        L.ref_final_score.restype = C.c_float
         L.ref_rng_flt.restype = None
         _lib = L
     return _lib
@@ -82,6 +84,13 @@ def replay(side: int, komi: float, moves):
     lib().ref_replay(side, komi, len(moves), _p(moves, C.c_int32),
                      _p(info, C.c_int32), _p(sq, C.c_uint8), _p(q, C.c_int32))
     return info, sq, q
+
+
+def final_score(side: int, komi: float, moves) -> float:
+    """FastBoard::final_mc_score after replaying `moves`."""
+    moves = np.ascontiguousarray(moves, np.int32)
+    return float(lib().ref_final_score(side, komi, len(moves),
+                                       _p(moves, C.c_int32)))
 
 
 def rng_u32(seed: int, n: int, max_: int = 0) -> np.ndarray:

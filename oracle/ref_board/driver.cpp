@@ -107,6 +107,15 @@ int ref_replay(int side, float komi, int n, const int32_t *moves,
     return 0;
 }
 
+// FastBoard::final_mc_score of the position after `moves`.
+float ref_final_score(int side, float komi, int n, const int32_t *moves) {
+    init_once();
+    FastState st;
+    st.init_game(side, komi);
+    for (int i = 0; i < n; i++) st.play_move(moves[i]);
+    return st.board.final_mc_score(komi);
+}
+
 void ref_rng_u32(uint32_t seed, int n, uint32_t max, uint32_t *out) {
     Random rng(seed);
     for (int i = 0; i < n; i++)
