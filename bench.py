@@ -25,6 +25,18 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0, "fp8": 5000.0}
 AGREEMENT_BAR = 0.995
 
+# HBM bytes per launch of the roofline kernel at batch 1024, measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE (one counter per run) and scaled by the
+# counters' reading for a known-size elementwise copy (reads x2.0, writes
+# x1.0); the summary with the commands is
+# profiles/fused_net_policy_hbm_traffic_b1024.md. Keyed by form.
+TRAFFIC_B1024 = {
+    "fp16": {"bytes": 56_680_000, "fetch_kib": 26203.7, "write_kib": 2944.0},
+    "bf16": {"bytes": 52_970_000, "fetch_kib": 24390.1, "write_kib": 2944.0},
+    "fp32": {"bytes": 579_100_000, "fetch_kib": 190213.0,
+             "write_kib": 185100.0},
+}
+
 
 def cpu_baseline(pol, val, planes, budget_s: float, threads: int):
     """Reference CPU path (oracle restatement: im2col + OpenBLAS sgemm +
@@ -216,7 +228,10 @@ def main() -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": (TRAFFIC_B1024[head]["bytes"] if B == 1024 else None),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + "
+                            "WRITE_SIZE x1.0, calibrated on a known-size "
+                            "copy)",
             "avg_launch_ms": round(avg_ms, 4),
         },
         "cpu_baseline": cpu,

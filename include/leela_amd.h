@@ -280,6 +280,54 @@ int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
                          int playouts, uint32_t playout_seed, int threads,
                          uint8_t *planes, float *winrates);
 
+/* ---- Learned playout surrogate -------------------------------------------
+ * A leaf win-rate estimator trained on playout outcomes, standing in for
+ * Playout::mc_owner (Playout.cpp:174-217) where a caller wants a playout
+ * estimate; the reference has no learned counterpart. Network: 3x3 conv
+ * 32->64 + relu, six residual blocks of two 3x3 64->64 convs, a 1x1 64->1
+ * conv + relu, FC 361->64 + relu, FC 64->1, tanh; win rate (1 + t) / 2
+ * for the side to move, from the value planes (lz_encode_planes with
+ * LZ_NET_VALUE). BF16 MFMA compute, FP32 master weights, Adam.
+ * Parameters are a flat FP32 vector: per conv W[64][cin][3][3] then
+ * b[64]; then wh[64], bh, F1[64][361], c1[64], F2[64], c2. */
+typedef struct lz_surrogate lz_surrogate;
+
+/* Allocates for batches up to max_batch on `device` (gfx950) and draws
+ * He-normal weights from seed (biases zero). */
+int lz_surrogate_create(lz_surrogate **out, int device, int max_batch,
+                        uint32_t seed);
+void lz_surrogate_destroy(lz_surrogate *s);
+int lz_surrogate_num_params(void);
+const char *lz_surrogate_last_error(const lz_surrogate *s);
+int lz_surrogate_get_params(lz_surrogate *s, float *params);
+/* Replaces the parameters and resets Adam (moments zero, step 0). */
+int lz_surrogate_set_params(lz_surrogate *s, const float *params);
+int lz_surrogate_set_adam(lz_surrogate *s, float beta1, float beta2,
+                          float eps);
+long long lz_surrogate_step(const lz_surrogate *s);
+
+/* Win rates [n] of device planes [n][32][361] (u8), on `stream` (NULL:
+ * the surrogate's own stream); asynchronous. */
+int lz_surrogate_forward_device(lz_surrogate *s, int n, const uint8_t *planes,
+                                float *winrate, void *stream);
+/* Host planes in, host win rates out; synchronous. */
+int lz_surrogate_predict(lz_surrogate *s, int n, const uint8_t *planes,
+                         float *winrate);
+/* Loss mean_n (t_n - (2 y_n - 1))^2 and its gradient (flat, num_params)
+ * for host planes and targets y (win rates); no update. grad or loss may
+ * be NULL. */
+int lz_surrogate_gradients(lz_surrogate *s, int n, const uint8_t *planes,
+                           const float *targets, float *grad, float *loss);
+/* One training step on device planes and targets: forward, backward,
+ * Adam at learning rate lr. If loss is not NULL the call waits and stores
+ * the step's loss (before the update). */
+int lz_surrogate_train_step_device(lz_surrogate *s, int n,
+                                   const uint8_t *planes, const float *targets,
+                                   float lr, float *loss, void *stream);
+/* Checkpoint: parameters, Adam moments, step and Adam settings. */
+int lz_surrogate_save(lz_surrogate *s, const char *path);
+int lz_surrogate_load(lz_surrogate *s, const char *path);
+
 /* Device name, as OpenCL::get_device_name (OpenCL.h:121). */
 const char *lz_device_name(lz_ctx *ctx);
 
