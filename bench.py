@@ -166,8 +166,20 @@ def main() -> int:
     value = h["pos_s"]
     ms_per_step = h["dt"] * 1e3 / h["steps"]
 
-    # Dominant kernel, timed per launch with HIP events on its stream.
+    # Host-buffer boundary (lz_forward: planes in from host memory, outputs
+    # back), policy + value per step: the PCIe-inclusive rate, reported
+    # beside the HBM-resident headline, never as `value`.
     p = leela_amd.PRECISIONS[head]
+    be.forward(leela_amd.NET_POLICY, p, planes)
+    be.forward(leela_amd.NET_VALUE, p, planes)
+    t0 = time.perf_counter()
+    host_steps = max(2, args.steps // 4)
+    for _ in range(host_steps):
+        be.forward(leela_amd.NET_POLICY, p, planes)
+        be.forward(leela_amd.NET_VALUE, p, planes)
+    host_pos_s = B * host_steps / (time.perf_counter() - t0)
+
+    # Dominant kernel, timed per launch with HIP events on its stream.
     kern_ms = []
     for _ in range(5):
         ms = be.forward_timed(leela_amd.NET_POLICY, p, B, d_planes.data_ptr(),
@@ -220,6 +232,7 @@ def main() -> int:
             "parallelism": "single GPU",
             "agreement_bar": AGREEMENT_BAR,
             "forms": forms_out,
+            "host_buffer_positions_per_s": round(host_pos_s, 1),
         },
         "roofline": {
             "bound": "mfma",

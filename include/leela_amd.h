@@ -324,6 +324,18 @@ int lz_surrogate_gradients(lz_surrogate *s, int n, const uint8_t *planes,
 int lz_surrogate_train_step_device(lz_surrogate *s, int n,
                                    const uint8_t *planes, const float *targets,
                                    float lr, float *loss, void *stream);
+/* Inspection of the last forward/backward pass, for layer-by-layer
+ * checks. kind 0: the BF16 output map [n][361][64] of conv `layer` (0..12,
+ * post-relu); kind 1: the BF16 gradient map at that conv's pre-activation,
+ * available once lz_surrogate_keep_maps(s, 1) has been set (it keeps all
+ * 13 maps instead of 3 rotating ones). Values are raw BF16 bit patterns.
+ * lz_surrogate_read_head copies the head's saved s [n][361], f [n][64] and
+ * t [n] (any pointer may be NULL). */
+int lz_surrogate_keep_maps(lz_surrogate *s, int on);
+int lz_surrogate_read_map(lz_surrogate *s, int kind, int layer, int n,
+                          uint16_t *out);
+int lz_surrogate_read_head(lz_surrogate *s, int n, float *sv, float *fv,
+                           float *tv);
 /* Checkpoint: parameters, Adam moments, step and Adam settings. */
 int lz_surrogate_save(lz_surrogate *s, const char *path);
 int lz_surrogate_load(lz_surrogate *s, const char *path);

@@ -52,6 +52,14 @@ def load_library():
         raise ImportError(
             f"{LIB_PATH} is missing: the HIP backend is not built "
             "(run `python -m leela_amd.build`)")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7. Once
+    # torch is loaded, the library's libamdhip64.so.7 dependency resolves to
+    # that copy; loaded the other way round, torch would bring up a second
+    # runtime and find no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     vp = C.c_void_p
     f32p = C.POINTER(C.c_float)

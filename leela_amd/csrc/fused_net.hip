@@ -67,6 +67,16 @@ __device__ __forceinline__ float elu(float v) {
     return v > 0.0f ? v : (expf(v) - 1.0f);
 }
 
+// ELU of a trunk output that is about to be rounded to a 16-bit operand:
+// exp(v) = 2^(v log2 e) on the hardware base-2 exponential (1 ulp), so
+// exp(v) - 1 carries the same ~1e-7 absolute error as with the library
+// expf, far below the 16-bit rounding that follows, without expf's
+// range reduction, which dominated the epilogue's VALU time.
+__device__ __forceinline__ float elu_operand(float v) {
+    return v > 0.0f ? v
+                    : __builtin_amdgcn_exp2f(v * 1.44269504088896341f) - 1.0f;
+}
+
 // One same-padded conv of the tile `act` (row pitch sp, stride PS, CIN
 // input channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the
 // packed weights) for the wave's column tiles; pb[t] is the lane's window
@@ -137,7 +147,8 @@ __device__ __forceinline__ void store_tile(T *act,
             if (po[t] < 0) continue;
             v4 v;
 #pragma unroll
-            for (int r = 0; r < 4; r++) v[r] = (T)elu(acc[m][t][r] + bv[r]);
+            for (int r = 0; r < 4; r++)
+                v[r] = (T)elu_operand(acc[m][t][r] + bv[r]);
             *reinterpret_cast<v4 *>(act + po[t] + o0) = v;
         }
     }

@@ -59,6 +59,7 @@ struct lz_surrogate {
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     uint16_t *wf[kNConv] = {}, *wb[kNConv] = {};
     uint16_t *act = nullptr, *e[3] = {};
+    uint16_t *ekeep = nullptr;   // all 13 gradient maps (inspection mode)
     float *s = nullptr, *f = nullptr, *t = nullptr, *y = nullptr;
     float *loss = nullptr, *part = nullptr;
     uint8_t *planes = nullptr;   // staging of the host entry points
@@ -78,6 +79,11 @@ struct lz_surrogate {
     }
     size_t map_elems() const { return (size_t)max_batch * kP * kCh; }
     uint16_t *act_of(int l) const { return act + (size_t)l * map_elems(); }
+    // Gradient map at the pre-activation of conv l. Three rotating maps
+    // suffice: computing map l-1 reads maps l and l+1.
+    uint16_t *emap(int l) const {
+        return ekeep ? ekeep + (size_t)l * map_elems() : e[l % 3];
+    }
 };
 
 namespace {
@@ -91,7 +97,8 @@ void release(lz_surrogate *s) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     void *bufs[] = {s->p, s->g, s->m, s->v, s->act, s->e[0], s->e[1], s->e[2],
-                    s->s, s->f, s->t, s->y, s->loss, s->part, s->planes, s->win};
+                    s->ekeep, s->s, s->f, s->t, s->y, s->loss, s->part,
+                    s->planes, s->win};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int l = 0; l < kNConv; l++) {
@@ -138,34 +145,28 @@ int forward(lz_surrogate *s, int n, const uint8_t *planes, float *win,
 }
 
 // Backward of the last forward into s->g (written, not accumulated); the
-// mean loss lands in s->loss (device).
+// mean loss lands in s->loss (device). Map l is the gradient at the
+// pre-activation of conv l: the second conv of a block (even l >= 2) passes
+// map l to its first conv, and the first conv (odd l) adds map l + 1 (the
+// skip) before masking by the block input.
 int backward(lz_surrogate *s, int n, const uint8_t *planes, const float *y,
              hipStream_t st) {
     const Layout &L = layout();
-    int cur = 0, resi = -1;
-    int rc = s->hip(head_bwd(s->act_of(kNConv - 1), s->p + L.head, s->s, s->f,
-                             s->t, y, s->e[cur], s->part, s->g + L.head,
-                             s->loss, n, st), "head_bwd");
-    if (rc) return rc;
-    for (int l = kNConv - 1; l >= 1; l--) {
-        rc = s->hip(wgrad(s->act_of(l - 1), kCh, s->e[cur], s->part,
+    const int last = kNConv - 1;
+    int rc = s->hip(head_bwd(s->act_of(last), s->p + L.head, s->s, s->f, s->t,
+                             y, s->emap(last), s->part, s->g + L.head, s->loss,
+                             n, st), "head_bwd");
+    for (int l = last; !rc && l >= 1; l--) {
+        rc = s->hip(wgrad(s->act_of(l - 1), kCh, s->emap(l), s->part,
                           s->g + L.w[l], s->g + L.b[l], n, st), "wgrad");
         if (rc) return rc;
-        int next = 0;
-        while (next == cur || next == resi) next++;
-        const uint16_t *res = nullptr;
-        if (l % 2 == 0) {
-            resi = cur;          // second conv of a block: keep for the skip
-        } else {
-            res = s->e[resi];    // first conv: add the skip's gradient
-        }
-        rc = s->hip(conv(s->e[cur], kCh, s->wb[l], nullptr, res, s->act_of(l - 1),
-                         s->e[next], n, true, st), "conv_bwd");
-        if (rc) return rc;
-        if (l % 2 == 1) resi = -1;
-        cur = next;
+        const uint16_t *res = (l % 2 == 1) ? s->emap(l + 1) : nullptr;
+        rc = s->hip(conv(s->emap(l), kCh, s->wb[l], nullptr, res,
+                         s->act_of(l - 1), s->emap(l - 1), n, true, st),
+                    "conv_bwd");
     }
-    return s->hip(wgrad(planes, 32, s->e[cur], s->part, s->g + L.w[0],
+    if (rc) return rc;
+    return s->hip(wgrad(planes, 32, s->emap(0), s->part, s->g + L.w[0],
                         s->g + L.b[0], n, st), "wgrad");
 }
 
@@ -383,6 +384,59 @@ int lz_surrogate_train_step_device(lz_surrogate *s, int n,
                                    hipMemcpyDeviceToHost, st), "copy loss");
     if (!rc) rc = join(s, st, s->stream);
     if (!rc && loss) rc = s->hip(hipStreamSynchronize(st), "sync");
+    return rc;
+}
+
+int lz_surrogate_keep_maps(lz_surrogate *s, int on) {
+    if (!s) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    (void)hipSetDevice(s->device);
+    int rc = s->hip(hipStreamSynchronize(s->stream), "sync");
+    if (rc) return rc;
+    if (on && !s->ekeep) {
+        if (dalloc(&s->ekeep, kNConv * s->map_elems()) != hipSuccess) {
+            s->ekeep = nullptr;
+            return s->fail(LZ_ERR_NOMEM, "gradient maps");
+        }
+    } else if (!on && s->ekeep) {
+        rc = s->hip(hipFree(s->ekeep), "free");
+        s->ekeep = nullptr;
+    }
+    return rc;
+}
+
+int lz_surrogate_read_map(lz_surrogate *s, int kind, int layer, int n,
+                          uint16_t *out) {
+    int rc = check(s, n);
+    if (rc) return rc;
+    if (!out || layer < 0 || layer >= kNConv || kind < 0 || kind > 1)
+        return LZ_ERR_ARG;
+    if (kind == 1 && !s->ekeep)
+        return s->fail(LZ_ERR_STATE, "gradient maps are not kept");
+    std::lock_guard<std::mutex> g(s->mu);
+    const uint16_t *src = kind == 0 ? s->act_of(layer) : s->emap(layer);
+    rc = s->hip(hipStreamSynchronize(s->stream), "sync");
+    if (!rc)
+        rc = s->hip(hipMemcpy(out, src, (size_t)n * kP * kCh * sizeof(uint16_t),
+                              hipMemcpyDeviceToHost), "copy map");
+    return rc;
+}
+
+int lz_surrogate_read_head(lz_surrogate *s, int n, float *sv, float *fv,
+                           float *tv) {
+    int rc = check(s, n);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(s->mu);
+    rc = s->hip(hipStreamSynchronize(s->stream), "sync");
+    if (!rc && sv)
+        rc = s->hip(hipMemcpy(sv, s->s, (size_t)n * kP * sizeof(float),
+                              hipMemcpyDeviceToHost), "copy s");
+    if (!rc && fv)
+        rc = s->hip(hipMemcpy(fv, s->f, (size_t)n * kHid * sizeof(float),
+                              hipMemcpyDeviceToHost), "copy f");
+    if (!rc && tv)
+        rc = s->hip(hipMemcpy(tv, s->t, (size_t)n * sizeof(float),
+                              hipMemcpyDeviceToHost), "copy t");
     return rc;
 }
 

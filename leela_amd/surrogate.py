@@ -43,6 +43,10 @@ def _lib():
         L.lz_surrogate_gradients.argtypes = [vp, C.c_int, u8p, f32p, f32p, f32p]
         L.lz_surrogate_train_step_device.argtypes = [vp, C.c_int, vp, vp,
                                                      C.c_float, f32p, vp]
+        L.lz_surrogate_keep_maps.argtypes = [vp, C.c_int]
+        L.lz_surrogate_read_map.argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(C.c_uint16)]
+        L.lz_surrogate_read_head.argtypes = [vp, C.c_int, f32p, f32p, f32p]
         L.lz_surrogate_save.argtypes = [vp, C.c_char_p]
         L.lz_surrogate_load.argtypes = [vp, C.c_char_p]
         L._surrogate_bound = True
@@ -151,6 +155,30 @@ class Surrogate:
             C.byref(loss) if want_loss else None, stream or None),
             "train_step")
         return loss.value if want_loss else None
+
+    def keep_maps(self, on: bool = True) -> None:
+        """Keep all 13 gradient maps of the next backward pass."""
+        self._chk(self._L.lz_surrogate_keep_maps(self._h, int(on)), "keep_maps")
+
+    def read_map(self, kind: int, layer: int, n: int) -> np.ndarray:
+        """Raw BF16 bits [n][361][64] of conv `layer`'s output (kind 0) or
+        of the gradient at its pre-activation (kind 1), from the last pass."""
+        out = np.empty((n, P, 64), np.uint16)
+        self._chk(self._L.lz_surrogate_read_map(
+            self._h, kind, layer, n, out.ctypes.data_as(C.POINTER(C.c_uint16))),
+            "read_map")
+        return out
+
+    def read_head(self, n: int):
+        """The head's saved (s [n][361], f [n][64], t [n])."""
+        sv = np.empty((n, P), np.float32)
+        fv = np.empty((n, 64), np.float32)
+        tv = np.empty(n, np.float32)
+        f32p = C.POINTER(C.c_float)
+        self._chk(self._L.lz_surrogate_read_head(
+            self._h, n, sv.ctypes.data_as(f32p), fv.ctypes.data_as(f32p),
+            tv.ctypes.data_as(f32p)), "read_head")
+        return sv, fv, tv
 
     def save(self, path) -> None:
         self._chk(self._L.lz_surrogate_save(self._h, os.fsencode(path)),

@@ -91,6 +91,57 @@ def _col2im(dcols, c):
     return xp[:, :, 1:-1, 1:-1]
 
 
+def conv_pre(x, W, b, res=None):
+    """Pre-activation of one conv: x [n][C][19][19], W [64][C][3][3]
+    (already rounded when emulating), b [64], res [n][64][361] or None.
+    Returns (z [n][64][361], im2col of x)."""
+    c = _cols(x)
+    z = np.einsum("ok,nkp->nop", W.reshape(W.shape[0], -1), c,
+                  optimize=True) + b[None, :, None]
+    if res is not None:
+        z = z + res
+    return z, c
+
+
+def conv_wgrad(e, cols, cin):
+    """dW [64][cin][3][3] and db [64] from the gradient e [n][64][361] at a
+    conv's pre-activation and the im2col of its input."""
+    dW = np.einsum("nop,nkp->ok", e, cols, optimize=True)
+    return dW.reshape(e.shape[1], cin, 3, 3), e.sum((0, 2))
+
+
+def conv_bwd_data(e, W):
+    """Gradient at a conv's input [n][cin][361] from e [n][64][361]."""
+    n, cin = e.shape[0], W.shape[1]
+    d = np.einsum("ok,nop->nkp", W.reshape(W.shape[0], -1), e, optimize=True)
+    return _col2im(d, cin).reshape(n, cin, P)
+
+
+def head_fwd(h6, p):
+    """s [n][361], f [n][64], t [n] from the last conv's output [n][64][361]."""
+    s = np.maximum(np.einsum("c,ncp->np", p["wh"], h6, optimize=True)
+                   + p["bh"][0], 0.0)
+    f = np.maximum(s @ p["F1"].T + p["c1"], 0.0)
+    t = np.tanh(f @ p["F2"] + p["c2"][0])
+    return s, f, t
+
+
+def head_bwd(h6, s, f, t, y, p):
+    """Head gradients {wh, bh, F1, c1, F2, c2} and ds [n][361] of the loss
+    mean_n (t - (2y - 1))^2 over the n rows given."""
+    n = t.shape[0]
+    y = np.asarray(y, np.float64)
+    dz = 2.0 * (t - (2.0 * y - 1.0)) / n * (1.0 - t * t)
+    g = {"F2": dz @ f, "c2": np.array([dz.sum()])}
+    df = np.outer(dz, p["F2"]) * (f > 0)
+    g["F1"] = df.T @ s
+    g["c1"] = df.sum(0)
+    ds = (df @ p["F1"]) * (s > 0)
+    g["wh"] = np.einsum("np,ncp->c", ds, h6, optimize=True)
+    g["bh"] = np.array([ds.sum()])
+    return g, ds
+
+
 def forward(flat, planes, bf16_points=False):
     """planes uint8 [n][32][361] -> (t [n], cache)."""
     p = unpack(np.asarray(flat, np.float64))
@@ -100,21 +151,15 @@ def forward(flat, planes, bf16_points=False):
     acts, cols = [], []
     h = None
     for layer in range(NCONV):
-        W = rnd(p[f"W{layer}"]).reshape(CH, -1)
-        c = _cols(x)
+        res = h.reshape(n, CH, P) if (layer >= 2 and layer % 2 == 0) else None
+        z, c = conv_pre(x, rnd(p[f"W{layer}"]), p[f"b{layer}"], res)
         cols.append(c)
-        z = np.einsum("ok,nkp->nop", W, c, optimize=True) + p[f"b{layer}"][None, :, None]
-        if layer >= 2 and layer % 2 == 0:      # second conv of a block
-            z = z + h.reshape(n, CH, P)
         a = rnd(np.maximum(z, 0.0)).reshape(n, CH, S, S)
         acts.append(a)
         if layer == 0 or layer % 2 == 0:
             h = a
         x = a
-    h6 = acts[-1].reshape(n, CH, P)
-    s = np.maximum(np.einsum("c,ncp->np", p["wh"], h6, optimize=True) + p["bh"][0], 0.0)
-    f = np.maximum(s @ p["F1"].T + p["c1"], 0.0)
-    t = np.tanh(f @ p["F2"] + p["c2"][0])
+    s, f, t = head_fwd(acts[-1].reshape(n, CH, P), p)
     cache = dict(p=p, planes=planes, acts=acts, cols=cols, s=s, f=f, t=t,
                  bf16=bf16_points)
     return t, cache
@@ -130,39 +175,24 @@ def backward(cache, y):
     rnd = bf16 if cache["bf16"] else (lambda v: v)
     s, f, t = cache["s"], cache["f"], cache["t"]
     n = t.shape[0]
-    g = {k: np.zeros_like(v) for k, v in p.items()}
-    y = np.asarray(y, np.float64)
-    dt = 2.0 * (t - (2.0 * y - 1.0)) / n
-    dz = dt * (1.0 - t * t)
-    g["F2"] = dz @ f
-    g["c2"][0] = dz.sum()
-    df = np.outer(dz, p["F2"]) * (f > 0)
-    g["F1"] = df.T @ s
-    g["c1"] = df.sum(0)
-    ds = (df @ p["F1"]) * (s > 0)
     h6 = acts[-1].reshape(n, CH, P)
-    g["wh"] = np.einsum("np,ncp->c", ds, h6, optimize=True)
-    g["bh"][0] = ds.sum()
-    # Gradient at the pre-activation of the last conv's output.
-    e = rnd(ds[:, None, :] * p["wh"][None, :, None] * (h6 > 0))
-    e_res = None
+    g, ds = head_bwd(h6, s, f, t, y, p)
+    # Map l: gradient at the pre-activation of conv l.
+    maps = [None] * NCONV
+    maps[-1] = rnd(ds[:, None, :] * p["wh"][None, :, None] * (h6 > 0))
     for layer in range(NCONV - 1, -1, -1):
         cin = 32 if layer == 0 else CH
-        g[f"W{layer}"] = np.einsum("nop,nkp->ok", e, cols[layer], optimize=True).reshape(
-            CH, cin, 3, 3)
-        g[f"b{layer}"] = e.sum((0, 2))
+        e = maps[layer]
+        g[f"W{layer}"], g[f"b{layer}"] = conv_wgrad(e, cols[layer], cin)
         if layer == 0:
             break
-        W = rnd(p[f"W{layer}"]).reshape(CH, -1)
-        dx = _col2im(np.einsum("ok,nop->nkp", W, e, optimize=True), CH).reshape(n, CH, P)
+        dx = conv_bwd_data(e, rnd(p[f"W{layer}"]))
+        if layer % 2 == 1:
+            # First conv of a block: add the skip's gradient (map l + 1).
+            dx = dx + maps[layer + 1]
         below = acts[layer - 1].reshape(n, CH, P)
-        if layer % 2 == 0:
-            # Into u_k (first conv of the block): keep e_k for the skip.
-            e_res = e
-            e = rnd(dx * (below > 0))
-        else:
-            # Into h_{k-1}: the block's conv path plus its skip.
-            e = rnd((dx + e_res) * (below > 0))
+        maps[layer - 1] = rnd(dx * (below > 0))
+    cache["maps"] = maps
     flat = np.concatenate([g[name].ravel() for name, _, _ in layout()])
     return loss_of(t, y), flat
 
