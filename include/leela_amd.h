@@ -40,7 +40,9 @@ extern "C" {
 
 #define LZ_PREC_FP32 0 /* plain form: FP32 everywhere, FP32 MFMA */
 #define LZ_PREC_BF16 1 /* BF16 trunk on MFMA, FP32 accumulate/epilogue */
-#define LZ_PREC_FP8 2  /* FP8 e4m3 trunk, per-output-channel weight scales */
+#define LZ_PREC_FP8 2  /* FP8 e4m3 operands after the 5x5 stem (FP16):
+                          per-output-channel weight scales, per-board
+                          activation scale per layer; 19x19 only */
 #define LZ_PREC_FP16 3 /* FP16 operands on the BF16 kernels' MFMA path:
                           same rate, 3 more mantissa bits */
 

@@ -176,6 +176,259 @@ __device__ float block_max(float v, float *s_red) {
     return r;
 }
 
+// ---- FP8 e4m3 trunk ---------------------------------------------------------
+// Fragments of v_mfma_scale_f32_16x16x128_f8f6f4 (format 0 = e4m3, unit
+// block scales): A lane holds 32 k-values of row lane%16, B lane 32 k-values
+// of column lane%16, lane group lane/16 the k-range 32*(lane/16)..+31 of
+// the 128-wide k-step (weights and activations use the same order).
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma_f8(v8i a, v8i b, f32x4 c) {
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, 0,
+                                                            127, 0, 127);
+}
+
+constexpr float kE4M3Max = 448.0f;
+
+// conv_tile on an e4m3 tile (pixel stride PS8 bytes): k = tap * CIN + c in
+// steps of 128; a lane's 32 k-values are 32 channels of one tap (CIN is a
+// multiple of 32). Past K = F*F*CIN the packed weights are zero and the
+// lane reads the last tap (finite values).
+template <int F, int CIN, int MT, int MTW, int PS8>
+__device__ __forceinline__ void conv_tile_f8(const uint8_t *act,
+                                             const v8i *__restrict__ wpk,
+                                             f32x4 (&acc)[MTW][kNT],
+                                             const int (&pb)[kNT], int sp,
+                                             int mt0, int lane) {
+    constexpr int STEPS = (F * F * CIN + 127) / 128;
+    const int g32 = (lane >> 4) * 32;
+#pragma unroll
+    for (int m = 0; m < MTW; m++)
+#pragma unroll
+        for (int t = 0; t < kNT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const v8i *wl = wpk + mt0 * 64 + lane;
+    // A fragments are prefetched one k-step ahead; the step loop stays
+    // rolled so that the compiler does not hoist every step's loads.
+    v8i a[MTW];
+#pragma unroll
+    for (int m = 0; m < MTW; m++) a[m] = wl[m * 64];
+#pragma unroll 1
+    for (int s = 0; s < STEPS; s++) {
+        const int k0 = s * 128 + g32;
+        int tap = k0 / CIN;
+        const int c0 = k0 - tap * CIN;
+        if (tap >= F * F) tap = F * F - 1;
+        const int off = ((tap / F) * sp + (tap % F)) * PS8 + c0;
+        const int nxt = s + 1 < STEPS ? s + 1 : s;
+        v8i an[MTW];
+#pragma unroll
+        for (int m = 0; m < MTW; m++) an[m] = wl[(nxt * MT + m) * 64];
+#pragma unroll
+        for (int t = 0; t < kNT; t++) {
+            const v8i b = *reinterpret_cast<const v8i *>(act + pb[t] + off);
+#pragma unroll
+            for (int m = 0; m < MTW; m++) acc[m][t] = mfma_f8(a[m], b, acc[m][t]);
+        }
+#pragma unroll
+        for (int m = 0; m < MTW; m++) a[m] = an[m];
+    }
+}
+
+// Epilogue of an e4m3 layer: v = ELU(acc * wscale[o] * s_in + bias[o]) in
+// FP32; the board's activation scale s_out = max|v| / 448 (a block
+// reduction: every thread must call this); e4m3(v / s_out) is written as
+// four bytes (four consecutive channels of one pixel) per accumulator.
+// Returns s_out. wscale == nullptr means unit weight scales.
+template <int MTW, int kWaves>
+__device__ __forceinline__ float store_tile_f8(uint8_t *act,
+                                               f32x4 (&acc)[MTW][kNT],
+                                               const float *__restrict__ bias,
+                                               const float *__restrict__ wscale,
+                                               float s_in, int mt0,
+                                               const int (&po)[kNT], int lane,
+                                               float *s_red) {
+    const int g4 = (lane >> 4) * 4;
+    float mx = 0.0f;
+#pragma unroll
+    for (int m = 0; m < MTW; m++) {
+        const int o0 = (mt0 + m) * 16 + g4;
+        float bv[4], sc[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            bv[r] = bias[o0 + r];
+            sc[r] = (wscale ? wscale[o0 + r] : 1.0f) * s_in;
+        }
+#pragma unroll
+        for (int t = 0; t < kNT; t++) {
+            if (po[t] < 0) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float v = elu_operand(acc[m][t][r] * sc[r] + bv[r]);
+                acc[m][t][r] = v;
+                mx = fmaxf(mx, fabsf(v));
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = s_red[0];
+    for (int i = 1; i < kWaves; i++) mx = fmaxf(mx, s_red[i]);
+    const float s_out = mx > 0.0f ? mx / kE4M3Max : 1.0f;
+    const float inv = 1.0f / s_out;
+#pragma unroll
+    for (int m = 0; m < MTW; m++) {
+        const int o0 = (mt0 + m) * 16 + g4;
+#pragma unroll
+        for (int t = 0; t < kNT; t++) {
+            if (po[t] < 0) continue;
+            float q[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                q[r] = fminf(fmaxf(acc[m][t][r] * inv, -kE4M3Max), kE4M3Max);
+            int packed = __builtin_amdgcn_cvt_pk_fp8_f32(q[0], q[1], 0, false);
+            packed = __builtin_amdgcn_cvt_pk_fp8_f32(q[2], q[3], packed, true);
+            *reinterpret_cast<int *>(act + po[t] + o0) = packed;
+        }
+    }
+    return s_out;
+}
+
+// The FP8 form: the 5x5 stem on FP16 operands (its 32 binary input planes
+// are exact in any format), every later conv (trunk, and the 1-channel head
+// conv) on e4m3 operands with per-output-channel weight scales
+// (args.ws[l]) and a per-board activation scale chosen after each layer
+// from the layer's output range. 19x19 boards only (no ragged slots).
+// Outputs as fused_net.
+template <int C, int M1, int NMID, bool VALUE, int kWM>
+__global__ __launch_bounds__(kWM * kWN * 64, 2) void fused_net_f8(
+    const uint8_t *__restrict__ planes, FusedArgs args,
+    float *__restrict__ raw, float *__restrict__ out, int nslots,
+    float temperature) {
+    typedef _Float16 T;
+    typedef typename Op<T>::v8 v8;
+    constexpr int PS8 = C + 16;
+    constexpr int STEM = 23 * 23 * kInPS * (int)sizeof(T);
+    constexpr int TRUNK = 21 * 21 * PS8;
+    constexpr int BYTES = STEM > TRUNK ? STEM : TRUNK;
+    constexpr int kWaves = kWM * kWN;
+    constexpr int kThreads = kWaves * 64;
+    constexpr int MT = C / 16, MTW = MT / kWM;
+    constexpr int MT1 = M1 / 16, MTW1 = MT1 / kWM;
+    static_assert(C % 32 == 0 && M1 % 32 == 0 && MT % kWM == 0 &&
+                      MT1 % kWM == 0, "channel blocking");
+
+    __shared__ __attribute__((aligned(16))) uint8_t s_raw[BYTES];
+    __shared__ float s_vec[384];
+    __shared__ float s_red[kWaves];
+    T *s_stem = reinterpret_cast<T *>(s_raw);
+    uint8_t *s_t8 = s_raw;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / kWN, wn = wave % kWN;
+    const int nt0 = wn * kNT;
+    constexpr int S = 19, SS = 361, pin = S + 4, pt = S + 2;
+    int pb_in[kNT], pb[kNT], po[kNT];
+#pragma unroll
+    for (int t = 0; t < kNT; t++) {
+        const int j = (nt0 + t) * 16 + (lane & 15);
+        const int jj = j < SS ? j : 0;
+        const int y = jj / S, x = jj % S;
+        pb_in[t] = (y * pin + x) * kInPS;
+        pb[t] = (y * pt + x) * PS8;
+        po[t] = j < SS ? pb[t] + (pt + 1) * PS8 : -1;
+    }
+
+    for (int board = blockIdx.x; board < nslots; board += gridDim.x) {
+        const uint8_t *src = planes + (size_t)board * 32 * kP;
+        for (int i = tid; i < pin * pin * 4; i += kThreads) {
+            const int pp = i >> 2, cg = (i & 3) * 8;
+            const int yy = pp / pin - 2, xx = pp % pin - 2;
+            const bool in = yy >= 0 && yy < S && xx >= 0 && xx < S;
+            v8 v;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                v[e] = (T)(in ? (float)src[(cg + e) * SS + yy * S + xx] : 0.0f);
+            *reinterpret_cast<v8 *>(s_stem + pp * kInPS + cg) = v;
+        }
+        __syncthreads();
+
+        float s_act;
+        {
+            f32x4 acc[MTW1][kNT];
+            conv_tile<T, 5, 32, MT1, MTW1, kInPS>(
+                s_stem, static_cast<const v8 *>(args.w[0]), acc, pb_in, pin,
+                wm * MTW1, lane);
+            __syncthreads();
+            for (int i = tid * 16; i < TRUNK; i += kThreads * 16)
+                *reinterpret_cast<v4i *>(s_t8 + i) = v4i{0, 0, 0, 0};
+            s_act = store_tile_f8<MTW1, kWaves>(s_t8, acc, args.b[0], nullptr,
+                                                1.0f, wm * MTW1, po, lane,
+                                                s_red);
+            __syncthreads();
+        }
+        {
+            f32x4 acc[MTW][kNT];
+            conv_tile_f8<3, M1, MT, MTW, PS8>(
+                s_t8, static_cast<const v8i *>(args.w[1]), acc, pb, pt,
+                wm * MTW, lane);
+            __syncthreads();
+            s_act = store_tile_f8<MTW, kWaves>(s_t8, acc, args.b[1], args.ws[1],
+                                               s_act, wm * MTW, po, lane, s_red);
+            __syncthreads();
+        }
+        for (int l = 2; l <= NMID; l++) {
+            f32x4 acc[MTW][kNT];
+            conv_tile_f8<3, C, MT, MTW, PS8>(
+                s_t8, static_cast<const v8i *>(args.w[l]), acc, pb, pt,
+                wm * MTW, lane);
+            __syncthreads();
+            s_act = store_tile_f8<MTW, kWaves>(s_t8, acc, args.b[l], args.ws[l],
+                                               s_act, wm * MTW, po, lane, s_red);
+            __syncthreads();
+        }
+        if (wm == 0) {
+            f32x4 acc[1][kNT];
+            conv_tile_f8<3, C, 1, 1, PS8>(
+                s_t8, static_cast<const v8i *>(args.w[NMID + 1]), acc, pb, pt,
+                0, lane);
+            const float hb = args.b[NMID + 1][0];
+            const float hs = args.ws[NMID + 1][0] * s_act;
+            if (lane < 16) {
+#pragma unroll
+                for (int t = 0; t < kNT; t++) {
+                    const int j = (nt0 + t) * 16 + lane;
+                    if (j < SS) s_vec[j] = elu(acc[0][t][0] * hs + hb);
+                }
+            }
+        }
+        __syncthreads();
+        if (VALUE) {
+            float *h = args.h0 + (size_t)board * kP;
+            for (int p = tid; p < kP; p += kThreads) h[p] = s_vec[p];
+        } else {
+            const size_t o = (size_t)board * kP;
+            float mx = -INFINITY;
+            for (int p = tid; p < SS; p += kThreads) mx = fmaxf(mx, s_vec[p]);
+            mx = block_max<kWaves>(mx, s_red);
+            const float alpha = mx / temperature;
+            float e[2] = {0.f, 0.f}, sum = 0.f;
+            for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
+                e[i] = expf(s_vec[p] / temperature - alpha);
+                sum += e[i];
+            }
+            sum = block_sum<kWaves>(sum, s_red);
+            for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
+                if (raw) raw[o + p] = s_vec[p];
+                if (out) out[o + p] = e[i] / sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // T: operand type; C: trunk width (128 policy, 64 value); M1: outputs of
 // the 5x5 stem; NMID: 3x3 convs between stem and head (the first takes M1
 // inputs). RAGGED: slots from args.slots (boards of any side up to 19,
@@ -376,6 +629,30 @@ hipError_t launch_wm(int which, const uint8_t *planes, const FusedArgs &args,
                          stream);
 }
 
+template <int WM>
+hipError_t launch_f8_wm(int which, const uint8_t *planes, const FusedArgs &args,
+                        float *raw, float *out, int n, float temperature,
+                        int grid, hipStream_t stream) {
+    const int threads = WM * kWN * 64;
+    if (which == 0) {
+        if (args.nconv != 13 || args.c != 128 || args.m1 != 96)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((fused_net_f8<128, 96, 11, false, WM>), dim3(grid),
+                           dim3(threads), 0, stream, planes, args, raw, out, n,
+                           temperature);
+        return hipGetLastError();
+    }
+    if (args.nconv != 12 || args.c != 64 || args.m1 != 64 || !args.h0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((fused_net_f8<64, 64, 10, true, WM>), dim3(grid),
+                       dim3(threads), 0, stream, planes, args, raw, out, n,
+                       temperature);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return value_fc_head(args.h0, args.fc1_wt, args.fc1_b, args.fc2_w,
+                         args.fc2_b, raw, out, n, kP, args.hidden, stream);
+}
+
 template <typename T, bool RAGGED>
 hipError_t launch(int which, int wm, const uint8_t *planes,
                   const FusedArgs &args, float *raw, float *out, int nslots,
@@ -429,7 +706,15 @@ hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
     const bool ragged = args.slots != nullptr;
     const int nslots = ragged ? args.nslots : n;
     const int wm = cfg.wm == 1 || cfg.wm == 2 ? cfg.wm : fused_default_wm(which);
-    const int grid = cfg.grid;
+    int grid = cfg.grid;
+    if (type == FusedType::kFP8) {
+        if (ragged) return hipErrorInvalidValue;
+        if (grid <= 0 || grid > n) grid = n;
+        return wm == 1 ? launch_f8_wm<1>(which, planes, args, raw, out, n,
+                                         temperature, grid, stream)
+                       : launch_f8_wm<2>(which, planes, args, raw, out, n,
+                                         temperature, grid, stream);
+    }
     if (type == FusedType::kBF16)
         return ragged ? launch<__bf16, true>(which, wm, planes, args, raw, out,
                                              nslots, n, temperature, grid,

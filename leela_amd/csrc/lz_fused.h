@@ -27,6 +27,9 @@ int ragged_slot_capacity(int side);
 struct FusedArgs {
     const void *w[kMaxFusedConvs];
     const float *b[kMaxFusedConvs];
+    // FP8 form: per-output-channel weight scales of conv l (l >= 1), the
+    // packed e4m3 weights being W / scale (see pack_conv_fp8).
+    const float *ws[kMaxFusedConvs] = {};
     const float *fc1_wt = nullptr;  // value head: [361][hidden]
     const float *fc1_b = nullptr;
     const float *fc2_w = nullptr;
@@ -45,7 +48,8 @@ struct FusedArgs {
 
 // MFMA operand type of the fused kernels: the conv weights are packed in
 // it and the LDS activation tile holds it.
-enum class FusedType { kBF16 = 0, kFP16 = 1 };
+// kFP8: FP16 stem, e4m3 operands for every later conv (19x19 only).
+enum class FusedType { kBF16 = 0, kFP16 = 1, kFP8 = 2 };
 
 // which: 0 policy (13 convs: 5x5 32->96, 3x3 96->128, 10x 128->128,
 // 128->1), 1 value (12 convs: 5x5 32->64, 10x 64->64, 64->1, + 2 FC).

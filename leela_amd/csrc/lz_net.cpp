@@ -2,6 +2,7 @@
 // layer schedule of the policy and value nets.
 #include "lz_net.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -160,6 +161,18 @@ int upload_net(Net &net, int which, std::string *msg) {
     }
     net.fused16 = A;
     for (int k = 0; k < nconv; k++) net.fused16.w[k] = net.layers[k].d_wf16;
+    net.fused8 = net.fused16;
+    for (int k = 1; k < nconv && e == hipSuccess; k++) {
+        std::vector<float> scales;
+        e = upload(&net.layers[k].d_wf8, pack_conv_fp8(L[k], &scales));
+        if (e == hipSuccess) e = upload(&net.layers[k].d_ws8, scales);
+        net.fused8.w[k] = net.layers[k].d_wf8;
+        net.fused8.ws[k] = net.layers[k].d_ws8;
+    }
+    if (e != hipSuccess) {
+        *msg = std::string("FP8 weight upload: ") + hipGetErrorString(e);
+        return LZ_ERR_HIP;
+    }
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
@@ -200,6 +213,56 @@ uint16_t float_to_f16_rne(float f) {
     return u;
 }
 
+uint8_t float_to_e4m3_rne(float f) {
+    if (std::isnan(f)) return 0x7f;
+    const uint8_t sign = std::signbit(f) ? 0x80 : 0x00;
+    const double a = std::fabs((double)f);
+    if (a >= 448.0) return sign | 0x7e;
+    if (a < 0.015625) {  // below 2^-6: subnormals, multiples of 2^-9
+        // nearbyint rounds half to even in the default rounding mode; 8
+        // becomes the smallest normal's code 0x08.
+        return sign | (uint8_t)std::nearbyint(a * 512.0);
+    }
+    int e = 0;
+    const double m = std::frexp(a, &e);   // a = m 2^e, m in [0.5, 1)
+    int E = e - 1;
+    int q = (int)std::nearbyint((2.0 * m - 1.0) * 8.0);
+    if (q == 8) {
+        q = 0;
+        E += 1;
+    }
+    int code = ((E + 7) << 3) | q;
+    if (code > 0x7e) code = 0x7e;
+    return sign | (uint8_t)code;
+}
+
+std::vector<uint8_t> pack_conv_fp8(const Layer &L, std::vector<float> *scales) {
+    const int F = L.filter, cin = L.cin, cout = L.cout;
+    const int K = F * F * cin, steps = (K + 127) / 128, MT = (cout + 15) / 16;
+    scales->assign((size_t)MT * 16, 1.0f);
+    for (int o = 0; o < cout; o++) {
+        float mx = 0.0f;
+        for (int k = 0; k < K; k++)
+            mx = std::max(mx, std::fabs(L.w[(size_t)o * K + k]));
+        if (mx > 0.0f) (*scales)[o] = mx / 448.0f;
+    }
+    std::vector<uint8_t> out((size_t)steps * MT * 64 * 32, 0);
+    size_t idx = 0;
+    for (int s = 0; s < steps; s++)
+        for (int m = 0; m < MT; m++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int j = 0; j < 32; j++, idx++) {
+                    const int k = s * 128 + (lane >> 4) * 32 + j;
+                    const int o = m * 16 + (lane & 15);
+                    if (k >= K || o >= cout) continue;
+                    const int tap = k / cin, c = k % cin;
+                    const float w = L.w[(((size_t)o * cin + c) * F + tap / F) * F +
+                                        tap % F];
+                    out[idx] = float_to_e4m3_rne(w / (*scales)[o]);
+                }
+    return out;
+}
+
 std::vector<uint16_t> pack_conv_fragments(const Layer &L, FusedType type) {
     const int F = L.filter, cin = L.cin, cout = L.cout;
     const int CB = cin / 32, steps = F * F * CB, MT = (cout + 15) / 16;
@@ -230,8 +293,12 @@ void free_net(Net &net) {
         dfree(L.d_b);
         dfree(L.d_wbf16);
         dfree(L.d_wf16);
+        dfree(L.d_wf8);
+        dfree(L.d_ws8);
         L.d_w32 = L.d_b = nullptr;
         L.d_wbf16 = L.d_wf16 = nullptr;
+        L.d_wf8 = nullptr;
+        L.d_ws8 = nullptr;
     }
 }
 
@@ -335,6 +402,27 @@ int run_net(const Net &net, int which, int precision, int n,
     if (precision == LZ_PREC_FP32)
         return run_fp32(net, which, n, d_planes, d_raw, d_out, temperature,
                         ws, stream, msg, timer);
+    if (precision == LZ_PREC_FP8) {
+        if (!net.fused_ok) {
+            *msg = "FP8 form implements only the reference layer shapes";
+            return LZ_ERR_UNSUPPORTED;
+        }
+        hipError_t e = timer ? timer->mark(stream) : hipSuccess;
+        FusedArgs args = net.fused8;
+        args.h0 = ws.h0;
+        // Default wave grid, one workgroup per board (the FP8 tile is
+        // smaller than the 16-bit one, so occupancy differs from theirs).
+        FusedLaunch cfg;
+        if (e == hipSuccess)
+            e = fused_forward(which, FusedType::kFP8, d_planes, args, d_raw,
+                              d_out, n, temperature, cfg, stream);
+        if (e == hipSuccess && timer) e = timer->mark(stream);
+        if (e != hipSuccess) {
+            *msg = std::string("FP8 forward launch: ") + hipGetErrorString(e);
+            return LZ_ERR_HIP;
+        }
+        return LZ_OK;
+    }
     if (precision == LZ_PREC_BF16 || precision == LZ_PREC_FP16) {
         if (!net.fused_ok) {
             *msg = "16-bit fused path implements only the reference layer "

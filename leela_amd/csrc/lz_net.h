@@ -26,6 +26,8 @@ struct Layer {
     float *d_b = nullptr;
     uint16_t *d_wbf16 = nullptr;  // BF16 MFMA fragments (fused kernels)
     uint16_t *d_wf16 = nullptr;   // FP16 MFMA fragments (fused kernels)
+    uint8_t *d_wf8 = nullptr;     // e4m3 MFMA fragments (FP8 form)
+    float *d_ws8 = nullptr;       // their per-output-channel scales
 };
 
 // A net as the reference pushes it (Network.cpp:206-233): a chain of convs
@@ -42,6 +44,7 @@ struct Net {
     bool fused_ok = false;  // shapes the fused kernels implement
     FusedArgs fused;        // device pointers for the BF16 fused kernel
     FusedArgs fused16;      // device pointers for the FP16 fused kernel
+    FusedArgs fused8;       // FP8 form: FP16 stem, e4m3 later convs
     int cus = 0;            // compute units of the device
     int occ[3] = {0, 0, 0}; // resident workgroups per CU for wm = 1, 2
     // Launch shape per batch bucket (<=64, <=256, <=1024, more), set by
@@ -108,6 +111,18 @@ int run_net(const Net &net, int which, int precision, int n,
 // channel block, lane = row + 16 * g holding input channels
 // 32*cb + 8*g .. +7.
 std::vector<uint16_t> pack_conv_fragments(const Layer &L, FusedType type);
+
+// OCP e4m3 ("E4M3FN": bias 7, max 448, no infinities) with round to
+// nearest even; magnitudes beyond 448 saturate to 448.
+uint8_t float_to_e4m3_rne(float f);
+
+// e4m3 fragments of v_mfma_scale_f32_16x16x128_f8f6f4: [k-step of 128]
+// [m-tile][lane][32 bytes], k = tap * cin + c, lane's bytes k = 128 s +
+// 32 (lane / 16) + j of output channel 16 m + lane % 16; zero past K and
+// past cout. Each output channel o is scaled by scales[o] = max |W[o]| /
+// 448 (1 for an all-zero channel) before rounding; scales has 16 * MT
+// entries.
+std::vector<uint8_t> pack_conv_fp8(const Layer &L, std::vector<float> *scales);
 uint16_t float_to_bf16_rne(float f);
 uint16_t float_to_f16_rne(float f);
 

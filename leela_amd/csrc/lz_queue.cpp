@@ -322,7 +322,7 @@ int lz_queue_start(lz_ctx *ctx, int precision, int max_batch,
         return ctx->fail(LZ_ERR_STATE, "queue before lz_finalize");
     if (ctx->queue) return ctx->fail(LZ_ERR_STATE, "queue already started");
     if (precision < LZ_PREC_FP32 || precision > LZ_PREC_FP16 ||
-        precision == LZ_PREC_FP8 || max_batch < 1 || max_batch > 2048 ||
+        max_batch < 1 || max_batch > 2048 ||
         max_wait_us < 0)
         return ctx->fail(LZ_ERR_ARG, "bad queue parameters");
     hipSetDevice(ctx->device);

@@ -206,3 +206,21 @@ def test_reference_shapes_f32_vs_f64(weights_b):
         r64, o64 = oracle.forward_f64(net, planes, head)
         assert np.abs(r32 - r64).max() < 1e-4
         assert np.abs(o32 - o64).max() < 1e-5
+
+
+def test_fp8_e4m3_rounding_matches_torch():
+    """The FP8 emulation's e4m3 rounding is torch's float8_e4m3fn (OCP
+    E4M3FN, round to nearest even) on every finite input range, with
+    saturation at 448."""
+    import torch
+    from oracle import fp8
+    rng = np.random.default_rng(8)
+    x = np.concatenate([
+        rng.normal(size=20000) * 2.0 ** rng.integers(-12, 9, 20000),
+        np.arange(-448, 449, 0.25),
+        [2.0 ** -9, 1.5 * 2.0 ** -10, 2.0 ** -10, 3 * 2.0 ** -10, 0.0,
+         2.0 ** -6 - 2.0 ** -10, 15.5 * 2.0 ** -4, 464.0, -464.0]])
+    x = x[np.abs(x) <= 448].astype(np.float32)
+    ref = torch.from_numpy(x).to(torch.float8_e4m3fn).float().numpy()
+    np.testing.assert_array_equal(fp8.e4m3(x).astype(np.float32), ref)
+    assert fp8.e4m3(1000.0) == 448.0 and fp8.e4m3(-1e9) == -448.0
