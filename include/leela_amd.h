@@ -287,7 +287,7 @@ int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
  * Playout::mc_owner (Playout.cpp:174-217) where a caller wants a playout
  * estimate; the reference has no learned counterpart. Network: 3x3 conv
  * 32->64 + relu, six residual blocks of two 3x3 64->64 convs, a 1x1 64->1
- * conv + relu, FC 361->64 + relu, FC 64->1, tanh; win rate (1 + t) / 2
+ * conv + ELU, FC 361->64 + ELU, FC 64->1, tanh; win rate (1 + t) / 2
  * for the side to move, from the value planes (lz_encode_planes with
  * LZ_NET_VALUE). BF16 MFMA compute, FP32 master weights, Adam.
  * Parameters are a flat FP32 vector: per conv W[64][cin][3][3] then
@@ -295,7 +295,9 @@ int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
 typedef struct lz_surrogate lz_surrogate;
 
 /* Allocates for batches up to max_batch on `device` (gfx950) and draws
- * He-normal weights from seed (biases zero). */
+ * He-normal weights from seed; biases, the second conv of every residual
+ * block and the output FC start at zero (each block starts as the
+ * identity, the output at win rate 0.5). */
 int lz_surrogate_create(lz_surrogate **out, int device, int max_batch,
                         uint32_t seed);
 void lz_surrogate_destroy(lz_surrogate *s);

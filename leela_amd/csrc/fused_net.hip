@@ -24,8 +24,9 @@
 //     per lane, prefetched one k-step ahead. The pixel-waves of a channel
 //     group walk the same k-steps together, so repeated fragment reads hit
 //     in L1 and each CU pulls a layer's weights from L2 once per slot.
-//   * Epilogue: bias + ELU in FP32 (Network.cpp:382-392), rounded to the
-//     operand type, written back over the input tile after a barrier. The
+//   * Bias (the accumulators' starting value) + ELU in FP32 (Network.cpp:
+//     382-392), rounded to the operand type in the epilogue and written
+//     back over the input tile after a barrier. The
 //     last 1-channel conv keeps FP32: softmax(T) per board over its logits
 //     (Network.cpp:450-469) for the policy net; for the value net the ELU'd
 //     plane goes to HBM and the batched inner-product head (value_fc_head)
@@ -86,15 +87,26 @@ __device__ __forceinline__ void conv_tile(const T *act,
                                           const typename Op<T>::v8 *__restrict__ wpk,
                                           f32x4 (&acc)[MTW][kNT],
                                           const int (&pb)[kNT], int sp,
-                                          int mt0, int lane) {
+                                          int mt0, int lane,
+                                          const float *__restrict__ bias =
+                                              nullptr) {
     typedef typename Op<T>::v8 v8;
     constexpr int CB = CIN / 32;
     constexpr int STEPS = F * F * CB;
     const int g8 = (lane >> 4) * 8;
+    // The accumulators start at the bias (when given): the epilogue then
+    // has one add less per output.
 #pragma unroll
-    for (int m = 0; m < MTW; m++)
+    for (int m = 0; m < MTW; m++) {
+        f32x4 init = {0.f, 0.f, 0.f, 0.f};
+        if (bias) {
+            const int o0 = (mt0 + m) * 16 + (lane >> 4) * 4;
 #pragma unroll
-        for (int t = 0; t < kNT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int r = 0; r < 4; r++) init[r] = bias[o0 + r];
+        }
+#pragma unroll
+        for (int t = 0; t < kNT; t++) acc[m][t] = init;
+    }
 
     const v8 *wl = wpk + mt0 * 64 + lane;
     v8 a[MTW], b[kNT];
@@ -124,31 +136,26 @@ __device__ __forceinline__ void conv_tile(const T *act,
     }
 }
 
-// Bias + ELU of acc, written into the trunk tile at the lane's pixel
-// offsets po[t] (-1: padding column). D layout of 16x16 MFMA: lane holds
-// rows 4*(lane/16)+r of column lane%16, i.e. four consecutive output
-// channels of one pixel.
+// ELU of acc (whose bias the conv folded into the accumulator start),
+// written into the trunk tile at the lane's pixel offsets po[t] (-1:
+// padding column). D layout of 16x16 MFMA: lane holds rows 4*(lane/16)+r
+// of column lane%16, i.e. four consecutive output channels of one pixel.
 template <typename T, int MTW>
 __device__ __forceinline__ void store_tile(T *act,
                                            const f32x4 (&acc)[MTW][kNT],
-                                           const float *__restrict__ bias,
-                                           int cout, int mt0,
-                                           const int (&po)[kNT], int lane) {
+                                           int mt0, const int (&po)[kNT],
+                                           int lane) {
     typedef typename Op<T>::v4 v4;
     const int g4 = (lane >> 4) * 4;
 #pragma unroll
     for (int m = 0; m < MTW; m++) {
         const int o0 = (mt0 + m) * 16 + g4;
-        float bv[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) bv[r] = o0 + r < cout ? bias[o0 + r] : 0.0f;
 #pragma unroll
         for (int t = 0; t < kNT; t++) {
             if (po[t] < 0) continue;
             v4 v;
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                v[r] = (T)elu_operand(acc[m][t][r] + bv[r]);
+            for (int r = 0; r < 4; r++) v[r] = (T)elu_operand(acc[m][t][r]);
             *reinterpret_cast<v4 *>(act + po[t] + o0) = v;
         }
     }
@@ -175,6 +182,34 @@ __device__ float block_max(float v, float *s_red) {
     for (int i = 1; i < kWaves; i++) r = fmaxf(r, s_red[i]);
     return r;
 }
+
+// Pixel -> (column tile, lane) map of a 19x19 board. A B fragment read is
+// one ds_read_b128 per lane; 16 lanes of one k-group are served together
+// and, with the pixel stride PS = C + 8 halves (4 mod 64 banks in dwords),
+// they are conflict-free iff their window origins (21 y + x in the haloed
+// tile) differ mod 16. Raster tiles of 16 pixels break that at every row
+// end (the halo adds 2); this map puts the g-th pixel of residue class
+// r = (21 y + x) mod 16 in tile g, lane r (each class has at most 23
+// members, so 23 tiles hold the board; -1 marks padding).
+struct PixelMap {
+    short p[24][16];
+};
+
+constexpr PixelMap make_pixel_map() {
+    PixelMap m{};
+    int count[16] = {};
+    for (int g = 0; g < 24; g++)
+        for (int r = 0; r < 16; r++) m.p[g][r] = -1;
+    for (int y = 0; y < 19; y++)
+        for (int x = 0; x < 19; x++) {
+            const int r = (21 * y + x) % 16;
+            m.p[count[r]][r] = (short)(y * 19 + x);
+            count[r]++;
+        }
+    return m;
+}
+
+__constant__ PixelMap kPixelMap = make_pixel_map();
 
 // ---- FP8 e4m3 trunk ---------------------------------------------------------
 // Fragments of v_mfma_scale_f32_16x16x128_f8f6f4 (format 0 = e4m3, unit
@@ -478,15 +513,22 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         }
         const int SS = S * S, npix = k * SS;
         const int pin = S + 4, pt = S + 2;  // stem / trunk row pitch
-        int pb_in[kNT], pb[kNT], po[kNT];
+        int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++) {
-            const int j = (nt0 + t) * 16 + (lane & 15);
-            const int jj = j < npix ? j : 0;
+            int j;
+            if (RAGGED) {
+                j = (nt0 + t) * 16 + (lane & 15);
+                if (j >= npix) j = -1;
+            } else {
+                j = kPixelMap.p[nt0 + t][lane & 15];
+            }
+            pix[t] = j;
+            const int jj = j >= 0 ? j : 0;
             const int kb = jj / SS, r = jj % SS, y = r / S, x = r % S;
             pb_in[t] = ((kb * (S + 2) + y) * pin + x) * kInPS;
             pb[t] = ((kb * (S + 1) + y) * pt + x) * PS;
-            po[t] = j < npix ? pb[t] + (pt + 1) * PS : -1;
+            po[t] = j >= 0 ? pb[t] + (pt + 1) * PS : -1;
         }
 
         // Stage the 32 binary planes with a 2-wide zero halo.
@@ -512,32 +554,32 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         {
             f32x4 acc[MTW1][kNT];
             conv_tile<T, 5, 32, MT1, MTW1, kInPS>(s_act, W(0), acc, pb_in,
-                                                  pin, wm * MTW1, lane);
+                                                  pin, wm * MTW1, lane,
+                                                  args.b[0]);
             __syncthreads();
             // Switch to the trunk geometry: clear tile and halos, then write.
             const int trunk = (1 + k * (S + 1)) * pt * PS;
             for (int i = tid * 8; i < trunk; i += kThreads * 8)
                 *reinterpret_cast<v8 *>(s_act + i) = v8{};
             __syncthreads();
-            store_tile<T, MTW1>(s_act, acc, args.b[0], M1, wm * MTW1, po,
-                                lane);
+            store_tile<T, MTW1>(s_act, acc, wm * MTW1, po, lane);
             __syncthreads();
         }
         // Middle 3x3 convs; the first reads M1 channels.
         {
             f32x4 acc[MTW][kNT];
             conv_tile<T, 3, M1, MT, MTW, PS>(s_act, W(1), acc, pb, pt,
-                                             wm * MTW, lane);
+                                             wm * MTW, lane, args.b[1]);
             __syncthreads();
-            store_tile<T, MTW>(s_act, acc, args.b[1], C, wm * MTW, po, lane);
+            store_tile<T, MTW>(s_act, acc, wm * MTW, po, lane);
             __syncthreads();
         }
         for (int l = 2; l <= NMID; l++) {
             f32x4 acc[MTW][kNT];
             conv_tile<T, 3, C, MT, MTW, PS>(s_act, W(l), acc, pb, pt,
-                                            wm * MTW, lane);
+                                            wm * MTW, lane, args.b[l]);
             __syncthreads();
-            store_tile<T, MTW>(s_act, acc, args.b[l], C, wm * MTW, po, lane);
+            store_tile<T, MTW>(s_act, acc, wm * MTW, po, lane);
             __syncthreads();
         }
         // Head: 3x3, C -> 1, in FP32; the channel-group-0 waves cover all
@@ -549,10 +591,8 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
             const float hb = args.b[NMID + 1][0];
             if (lane < 16) {
 #pragma unroll
-                for (int t = 0; t < kNT; t++) {
-                    const int j = (nt0 + t) * 16 + lane;
-                    if (j < npix) s_vec[j] = elu(acc[0][t][0] + hb);
-                }
+                for (int t = 0; t < kNT; t++)
+                    if (pix[t] >= 0) s_vec[pix[t]] = elu(acc[0][t][0] + hb);
             }
         }
         __syncthreads();

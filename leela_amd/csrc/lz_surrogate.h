@@ -3,8 +3,8 @@
 //
 //   a0 = relu(conv3x3(planes) + b0)                      32 -> 64
 //   block k = 1..6: u = relu(conv(h) + b), h' = relu(conv(u) + b + h)
-//   s  = relu(wh . h6 + bh)   (1x1, 64 -> 1)
-//   f  = relu(F1 s + c1)      (361 -> 64)
+//   s  = elu(wh . h6 + bh)    (1x1, 64 -> 1)
+//   f  = elu(F1 s + c1)       (361 -> 64)
 //   t  = tanh(F2 . f + c2),   win rate (1 + t) / 2
 //
 // Activations and conv gradients live in HBM as BF16 HWC maps [n][361][64];

@@ -24,7 +24,8 @@
 //   each workgroup writes its FP32 partial, and a second kernel sums the
 //   chunks in a fixed order (deterministic) into OIHW.
 //
-// Head, loss, Adam: FP32, small kernels (the head is 0.2% of the FLOPs).
+// Head (ELU activations), loss, Adam: FP32, small kernels (the head is
+// 0.2% of the FLOPs).
 #include "lz_surrogate.h"
 
 namespace lz {
@@ -46,6 +47,13 @@ constexpr int kMaxChunks = 64;
 __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
+
+// The head's activation: ELU, as in the reference value head
+// (Network.cpp:676-740). A ReLU on the 1-channel plane can switch off for
+// every pixel and stop all learning; ELU keeps a gradient everywhere.
+__device__ inline float elu(float v) { return v > 0.f ? v : expf(v) - 1.f; }
+// d ELU / d v expressed through the output a = ELU(v).
+__device__ inline float elu_grad(float a) { return a > 0.f ? 1.f : a + 1.f; }
 
 __device__ inline float wave_sum(float v) {
 #pragma unroll
@@ -305,7 +313,7 @@ __global__ __launch_bounds__(kThreads) void sg_head_fwd(
 #pragma unroll
             for (int k = 0; k < 8; k++) d += wh[c8 * 8 + k] * (float)v[k];
         }
-        d = fmaxf(d, 0.f);
+        d = elu(d);
         s[p] = d;
         s_out[(size_t)b * P + p] = d;
     }
@@ -316,7 +324,7 @@ __global__ __launch_bounds__(kThreads) void sg_head_fwd(
         for (int p = lane; p < P; p += 64) d += F1[j * P + p] * s[p];
         d = wave_sum(d);
         if (lane == 0) {
-            d = fmaxf(d + c1[j], 0.f);
+            d = elu(d + c1[j]);
             f[j] = d;
             f_out[(size_t)b * kHid + j] = d;
         }
@@ -369,7 +377,7 @@ __global__ __launch_bounds__(kThreads, 1) void sg_head_bwd(
         }
         if (tid < kHid) {
             const float fj = f_in[(size_t)b * kHid + tid];
-            const float d = fj > 0.f ? dz * F2[tid] : 0.f;
+            const float d = dz * F2[tid] * elu_grad(fj);
             df[tid] = d;
             aF2 += dz * fj;
             ac1 += d;
@@ -384,7 +392,7 @@ __global__ __launch_bounds__(kThreads, 1) void sg_head_bwd(
         for (int p = tid; p < P; p += kThreads) {
             float d = 0.f;
             for (int j = 0; j < kHid; j++) d += F1[j * P + p] * df[j];
-            ds[p] = s[p] > 0.f ? d : 0.f;
+            ds[p] = d * elu_grad(s[p]);
         }
         __syncthreads();
         for (int p = g; p < P; p += 4) {

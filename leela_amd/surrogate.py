@@ -191,11 +191,12 @@ class Surrogate:
 
 def train(sur: Surrogate, planes, labels, steps: int, batch: int = 512,
           lr: float = 1e-3, seed: int = 0, log_every: int = 0,
-          device: str = "cuda:0"):
+          warmup: int = 200, device: str = "cuda:0"):
     """Minibatch Adam training on planes [N][32][361] and win-rate labels
     [N], both copied once to device memory. Minibatches are drawn without
     replacement within each pass over the data (shuffled per pass from
-    `seed`). Returns the per-step losses."""
+    `seed`); the learning rate ramps linearly to `lr` over the surrogate's
+    first `warmup` steps. Returns [(step, loss)] every `log_every` steps."""
     import torch
     dev = torch.device(device)
     X = torch.from_numpy(np.ascontiguousarray(planes, np.uint8)).to(dev)
@@ -220,7 +221,9 @@ def train(sur: Surrogate, planes, labels, steps: int, batch: int = 512,
         torch.index_select(X, 0, idx, out=xb)
         torch.index_select(Y, 0, idx, out=yb)
         want = bool(log_every) and (it % log_every == 0 or it == steps - 1)
-        loss = sur.train_step_device(batch, xb.data_ptr(), yb.data_ptr(), lr,
+        k = sur.step + 1
+        lr_k = lr * min(1.0, k / warmup) if warmup > 0 else lr
+        loss = sur.train_step_device(batch, xb.data_ptr(), yb.data_ptr(), lr_k,
                                      want_loss=want, stream=stream)
         if want:
             losses.append((it, loss))

@@ -9,7 +9,8 @@ Training positions are generator positions from seeds seed0..seed0+n-1
 `--playouts` playouts each (playouts of position i seeded
 playout_seed0 + i), read from `--labels` (a float32 .npy of n win rates)
 when given, generated otherwise. Training is Adam on minibatches of
-`--batch`, with a checkpoint every `--ckpt-every` steps and at the end.
+`--batch` (learning rate ramped over `--warmup` steps), with a checkpoint
+every `--ckpt-every` steps and at the end.
 
 The evaluation labels held-out positions (seeds eval_seed0..) with a fresh
 `--eval-playouts` mean and reports the mean absolute error of the
@@ -38,6 +39,7 @@ def main(argv=None) -> int:
     ap.add_argument("--epochs", type=float, default=2.0)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--ckpt", default=None)
     ap.add_argument("--ckpt-every", type=int, default=1000)
@@ -86,7 +88,8 @@ def main(argv=None) -> int:
         k = min(args.ckpt_every, steps - done)
         losses += [(done + i, l) for i, l in
                    train(sur, planes, labels, k, batch=args.batch, lr=args.lr,
-                         seed=args.seed + done, log_every=args.log_every)]
+                         seed=args.seed + done, log_every=args.log_every,
+                         warmup=args.warmup)]
         done += k
         if args.ckpt:
             sur.save(args.ckpt)

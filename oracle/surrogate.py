@@ -8,8 +8,8 @@ the network's definition, not a reference routine:
     block k = 1..6:
         u_k = relu(conv3x3(h_{k-1}; W_{2k-1}) + b_{2k-1})
         h_k = relu(conv3x3(u_k; W_{2k}) + b_{2k} + h_{k-1})   (h_0 = a0)
-    s   = relu(sum_c wh[c] h_6[c, p] + bh)                    1x1, 64 -> 1
-    f   = relu(F1 s + c1)                                     361 -> 64
+    s   = elu(sum_c wh[c] h_6[c, p] + bh)                     1x1, 64 -> 1
+    f   = elu(F1 s + c1)                                      361 -> 64
     t   = tanh(F2 . f + c2)                                   64 -> 1
     win = (1 + t) / 2
     loss = mean_n (t_n - (2 y_n - 1))^2
@@ -117,11 +117,19 @@ def conv_bwd_data(e, W):
     return _col2im(d, cin).reshape(n, cin, P)
 
 
+def _elu(v):
+    return np.where(v > 0.0, v, np.expm1(np.minimum(v, 0.0)))
+
+
+def _elu_grad(a):
+    """d ELU / d v through the output a = ELU(v)."""
+    return np.where(a > 0.0, 1.0, a + 1.0)
+
+
 def head_fwd(h6, p):
     """s [n][361], f [n][64], t [n] from the last conv's output [n][64][361]."""
-    s = np.maximum(np.einsum("c,ncp->np", p["wh"], h6, optimize=True)
-                   + p["bh"][0], 0.0)
-    f = np.maximum(s @ p["F1"].T + p["c1"], 0.0)
+    s = _elu(np.einsum("c,ncp->np", p["wh"], h6, optimize=True) + p["bh"][0])
+    f = _elu(s @ p["F1"].T + p["c1"])
     t = np.tanh(f @ p["F2"] + p["c2"][0])
     return s, f, t
 
@@ -133,10 +141,10 @@ def head_bwd(h6, s, f, t, y, p):
     y = np.asarray(y, np.float64)
     dz = 2.0 * (t - (2.0 * y - 1.0)) / n * (1.0 - t * t)
     g = {"F2": dz @ f, "c2": np.array([dz.sum()])}
-    df = np.outer(dz, p["F2"]) * (f > 0)
+    df = np.outer(dz, p["F2"]) * _elu_grad(f)
     g["F1"] = df.T @ s
     g["c1"] = df.sum(0)
-    ds = (df @ p["F1"]) * (s > 0)
+    ds = (df @ p["F1"]) * _elu_grad(s)
     g["wh"] = np.einsum("np,ncp->c", ds, h6, optimize=True)
     g["bh"] = np.array([ds.sum()])
     return g, ds

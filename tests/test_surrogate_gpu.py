@@ -26,11 +26,45 @@ from leela_amd.surrogate import Surrogate, LeafEvaluator, train
 pytestmark = pytest.mark.gpu
 
 
+def _random_params(seed):
+    """Every tensor drawn nonzero (the library starts the residual blocks'
+    second convs and the output layer at zero, which would leave parts of
+    the gradient identically zero)."""
+    rng = np.random.default_rng(seed)
+    flat = np.zeros(orc.num_params(), np.float32)
+    for name, shape, off in orc.layout():
+        n = int(np.prod(shape))
+        if name.startswith("W"):
+            std = np.sqrt(2.0 / (shape[1] * 9))
+        elif name == "F1":
+            std = np.sqrt(2.0 / orc.P)
+        elif name in ("wh", "F2"):
+            std = np.sqrt(1.0 / 64)
+        else:
+            std = 0.05
+        flat[off:off + n] = rng.normal(0, std, n)
+    return flat
+
+
 @pytest.fixture(scope="module")
 def sur():
     s = Surrogate(0, max_batch=512, seed=3)
+    s.set_params(_random_params(3))
     yield s
     s.close()
+
+
+def test_initialisation_starts_blocks_as_identity():
+    s = Surrogate(0, max_batch=8, seed=5)
+    try:
+        p = orc.unpack(s.params().astype(np.float64))
+    finally:
+        s.close()
+    for layer in range(orc.NCONV):
+        zero = layer >= 2 and layer % 2 == 0
+        assert (np.abs(p[f"W{layer}"]).max() == 0) == zero, layer
+        assert np.abs(p[f"b{layer}"]).max() == 0
+    assert np.abs(p["F2"]).max() == 0 and np.abs(p["F1"]).max() > 0
 
 
 def _value_planes(n, seed0):
@@ -101,6 +135,7 @@ def test_kernels_layer_by_layer():
     n = 6
     s = Surrogate(0, max_batch=16, seed=11)
     try:
+        s.set_params(_random_params(11))
         s.keep_maps(True)
         planes = _value_planes(n, 1500)
         y = np.random.default_rng(3).uniform(0.1, 0.9, n).astype(np.float32)
@@ -165,6 +200,7 @@ def test_adam_step_matches_oracle():
     print("surrogate adam: 2 steps on 8 positions", flush=True)
     s = Surrogate(0, max_batch=64, seed=9)
     try:
+        s.set_params(_random_params(9))
         planes = _value_planes(8, 900)
         y = np.linspace(0.2, 0.8, 8).astype(np.float32)
         X = torch.from_numpy(planes).cuda()
