@@ -38,26 +38,45 @@ TRAFFIC_B1024 = {
 }
 
 
-def cpu_baseline(pol, val, planes, budget_s: float, threads: int):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pol, val, planes, vplanes, budget_s: float, threads: int):
     """Reference CPU path (oracle restatement: im2col + OpenBLAS sgemm +
-    bias + ELU, FP32) on a bounded sample of the same positions."""
+    bias + ELU, FP32) on a bounded sample of the same positions, with
+    `threads` workers x 1 BLAS thread (Network.cpp:236-250) and with one."""
     import oracle
     blas = oracle.load_blas()
     pnet, vnet = oracle.Net(pol), oracle.Net(val)
-    n = min(len(planes), 2 * threads)
-    t0 = time.perf_counter()
-    oracle.forward_f32(pnet, planes[:n], oracle.HEAD_POLICY, threads=threads)
-    oracle.forward_f32(vnet, planes[:n], oracle.HEAD_VALUE, threads=threads)
-    per = (time.perf_counter() - t0) / n
-    m = int(max(n, min(len(planes), budget_s / per)))
-    t0 = time.perf_counter()
-    oracle.forward_f32(pnet, planes[:m], oracle.HEAD_POLICY, threads=threads)
-    oracle.forward_f32(vnet, planes[:m], oracle.HEAD_VALUE, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(m / dt, 2), "unit": "positions/s", "cores": threads,
+
+    def rate(t, budget):
+        n = min(len(planes), 2 * t)
+        t0 = time.perf_counter()
+        oracle.forward_f32(pnet, planes[:n], oracle.HEAD_POLICY, threads=t)
+        oracle.forward_f32(vnet, vplanes[:n], oracle.HEAD_VALUE, threads=t)
+        per = (time.perf_counter() - t0) / n
+        m = int(max(n, min(len(planes), budget / per)))
+        t0 = time.perf_counter()
+        oracle.forward_f32(pnet, planes[:m], oracle.HEAD_POLICY, threads=t)
+        oracle.forward_f32(vnet, vplanes[:m], oracle.HEAD_VALUE, threads=t)
+        return m, m / (time.perf_counter() - t0)
+
+    m, v = rate(threads, budget_s)
+    m1, v1 = rate(1, budget_s / 3)
+    return {"value": round(v, 2), "unit": "positions/s", "cores": threads,
             "kind": "port",
             "sample": f"{m} of the benchmark's positions, policy+value, "
-                      f"{threads} worker threads x 1 BLAS thread, {blas}"}
+                      f"{threads} worker threads x 1 BLAS thread, {blas}",
+            "value_1_thread": round(v1, 2), "sample_1_thread": m1,
+            "cpu_model": _cpu_model()}
 
 
 def main() -> int:
@@ -96,19 +115,22 @@ def main() -> int:
     be.finalize()
 
     planes = positions.batch_planes(B, net=0, seed0=1)
+    vplanes = positions.batch_planes(B, net=1, seed0=1)
     dev = torch.device("cuda:0")
     d_planes = torch.from_numpy(planes).to(dev)
+    d_vplanes = torch.from_numpy(vplanes).to(dev)
     d_praw = torch.empty((B, 361), dtype=torch.float32, device=dev)
     d_pout = torch.empty((B, 361), dtype=torch.float32, device=dev)
     d_vraw = torch.empty((B, 1), dtype=torch.float32, device=dev)
     d_vout = torch.empty((B, 1), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
 
-    def step(p):
+    def step(p, st=None):
+        st = stream if st is None else st
         be.forward_device(leela_amd.NET_POLICY, p, B, d_planes.data_ptr(),
-                          d_praw.data_ptr(), d_pout.data_ptr(), stream)
-        be.forward_device(leela_amd.NET_VALUE, p, B, d_planes.data_ptr(),
-                          d_vraw.data_ptr(), d_vout.data_ptr(), stream)
+                          d_praw.data_ptr(), d_pout.data_ptr(), st)
+        be.forward_device(leela_amd.NET_VALUE, p, B, d_vplanes.data_ptr(),
+                          d_vraw.data_ptr(), d_vout.data_ptr(), st)
 
     def run(name, steps, warmup):
         p = leela_amd.PRECISIONS[name]
@@ -144,7 +166,7 @@ def main() -> int:
         be.close()
         return 0
 
-    forms = ["fp32", "bf16", "fp16"]
+    forms = ["fp32", "bf16", "fp16", "fp8"]
     res = {}
     for name in forms:
         steps = args.steps if name != "fp32" else max(2, args.steps // 4)
@@ -171,13 +193,51 @@ def main() -> int:
     # beside the HBM-resident headline, never as `value`.
     p = leela_amd.PRECISIONS[head]
     be.forward(leela_amd.NET_POLICY, p, planes)
-    be.forward(leela_amd.NET_VALUE, p, planes)
+    be.forward(leela_amd.NET_VALUE, p, vplanes)
     t0 = time.perf_counter()
     host_steps = max(2, args.steps // 4)
     for _ in range(host_steps):
         be.forward(leela_amd.NET_POLICY, p, planes)
-        be.forward(leela_amd.NET_VALUE, p, planes)
+        be.forward(leela_amd.NET_VALUE, p, vplanes)
     host_pos_s = B * host_steps / (time.perf_counter() - t0)
+
+    # The policy + value step captured once as a hipGraph and replayed:
+    # per-replay HIP events, median of 50 after 10 warm-up replays.
+    graph_pos_s = None
+    try:
+        gs = torch.cuda.Stream(dev)
+        with torch.cuda.stream(gs):
+            for _ in range(2):
+                step(p, gs.cuda_stream)
+        torch.cuda.synchronize(dev)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=gs):
+            step(p, gs.cuda_stream)
+        times = []
+        for i in range(60):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            graph.replay()
+            e1.record()
+            e1.synchronize()
+            if i >= 10:
+                times.append(e0.elapsed_time(e1))
+        graph_pos_s = B / (float(np.median(times)) * 1e-3)
+        del graph
+    except RuntimeError as err:  # capture unsupported: report, do not fail
+        print(f"graph replay skipped: {err}", file=sys.stderr)
+
+    # Host feature encoder (board engine + plane encoders, both nets) on
+    # the same host threads: the end-to-end rate is bounded by the slower
+    # of the encoder and the device when the two overlap.
+    enc_threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    positions.make_batch(B, seed0=1, net=leela_amd.NET_POLICY,
+                         threads=enc_threads)
+    positions.make_batch(B, seed0=1, net=leela_amd.NET_VALUE,
+                         threads=enc_threads)
+    enc_pos_s = B / (time.perf_counter() - t0)
 
     # Dominant kernel, timed per launch with HIP events on its stream.
     kern_ms = []
@@ -199,7 +259,8 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(pol, val, planes, args.cpu_seconds, threads)
+        cpu = cpu_baseline(pol, val, planes, vplanes, args.cpu_seconds,
+                           threads)
 
     forms_out = {
         n: {"positions_per_s": round(res[n]["pos_s"], 1),
@@ -233,6 +294,11 @@ def main() -> int:
             "agreement_bar": AGREEMENT_BAR,
             "forms": forms_out,
             "host_buffer_positions_per_s": round(host_pos_s, 1),
+            "graph_replay_positions_per_s": (round(graph_pos_s, 1)
+                                             if graph_pos_s else None),
+            "encoder_positions_per_s": round(enc_pos_s, 1),
+            "encoder_threads": enc_threads,
+            "end_to_end_positions_per_s": round(min(enc_pos_s, value), 1),
         },
         "roofline": {
             "bound": "mfma",

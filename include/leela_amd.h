@@ -224,6 +224,15 @@ int lz_encode_planes(const lz_position *pos, int net, int symmetry,
 #define LZ_QUERY_FIELDS 12
 int lz_board_queries(const lz_position *pos, int32_t *out);
 
+/* The encoder over a batch: planes of n positions of one board side,
+ * uint8 [n][32][S*S], each as lz_encode_planes(pos[i], net, symmetry), on
+ * `threads` host threads (<= 0: every hardware thread). The per-leaf
+ * gather_features_policy / gather_features_value calls of
+ * async_scored_moves, get_value and get_scored_moves (Network.cpp:557,
+ * 597, 633), batched. */
+int lz_encode_batch(const lz_position *const *pos, int n, int net,
+                    int symmetry, int threads, uint8_t *planes);
+
 /* Generator + encoder for n positions with seeds seed0..seed0+n-1:
  * planes uint8 [n][32][side*side] of `net`, rotation 0. threads <= 0 uses
  * every hardware thread. */

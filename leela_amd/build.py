@@ -36,6 +36,22 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# Sources with no device code and no HIP runtime calls (the board engine,
+# encoders, playouts and their C ABI): built by the host C++ compiler.
+HOST_ONLY = {"go_board.cpp", "input_planes.cpp", "playout.cpp",
+             "lz_host_abi.cpp"}
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-mpopcnt", "-Wall",
+              f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+
+def host_cxx() -> str:
+    for cand in (os.environ.get("CXX"), shutil.which("g++"),
+                 "/opt/rocm/llvm/bin/clang++"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("no host C++ compiler")
+
+
 def sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
 
@@ -43,12 +59,15 @@ def sources() -> list[Path]:
 def _compile(src: Path) -> Path:
     OBJ.mkdir(parents=True, exist_ok=True)
     obj = OBJ / (src.name + ".o")
-    cmd = [hipcc(), *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
-    if src.suffix == ".cpp":
-        cmd[1:1] = ["-x", "hip"]
+    if src.name in HOST_ONLY:
+        cmd = [host_cxx(), *HOST_FLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [hipcc(), *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
+        if src.suffix == ".cpp":
+            cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr}")
+        raise RuntimeError(f"compile failed on {src.name}:\n{r.stderr}")
     return obj
 
 

@@ -228,6 +228,128 @@ std::pair<int, int> Board::after_liberties(int c, int v) const {
     return {after_liberties_colour(c, v), after_liberties_colour(other(c), v)};
 }
 
+namespace {
+
+using Set = BoardBits::Set;
+constexpr int kWords = BoardBits::kWords;
+
+inline void set_bit(Set &s, int i) { s.w[i >> 6] |= 1ull << (i & 63); }
+
+inline void or_into(Set &d, const Set &s) {
+    for (int i = 0; i < kWords; i++) d.w[i] |= s.w[i];
+}
+
+// d = s shifted towards higher (k > 0) or lower (k < 0) indices by |k| < 64.
+inline void or_shifted(Set &d, const Set &s, int k) {
+    if (k > 0) {
+        uint64_t carry = 0;
+        for (int i = 0; i < kWords; i++) {
+            d.w[i] |= (s.w[i] << k) | carry;
+            carry = s.w[i] >> (64 - k);
+        }
+    } else {
+        const int m = -k;
+        uint64_t carry = 0;
+        for (int i = kWords - 1; i >= 0; i--) {
+            d.w[i] |= (s.w[i] >> m) | carry;
+            carry = s.w[i] << (64 - m);
+        }
+    }
+}
+
+}  // namespace
+
+namespace {
+
+// The 4-neighbours of s (s itself excluded only where it has no neighbour
+// in s).
+inline Set neighbours(const Set &s, int W) {
+    Set n{};
+    or_shifted(n, s, 1);
+    or_shifted(n, s, -1);
+    or_shifted(n, s, W);
+    or_shifted(n, s, -W);
+    return n;
+}
+
+inline int count_and(const Set &a, const Set &b) {
+    int n = 0;
+    for (int i = 0; i < kWords; i++) n += __builtin_popcountll(a.w[i] & b.w[i]);
+    return n;
+}
+
+}  // namespace
+
+BoardBits::BoardBits(const Board &b) : b_(b) {
+    const int W = b.width();
+    int heads[kMaxSquares];
+    int nh = 0;
+    for (int v = W; v < W * (W - 1); v++) {
+        const int c = b.at(v);
+        if (c == EMPTY) {
+            set_bit(empty_, v);
+        } else if (c == BLACK || c == WHITE) {
+            const int h = b.head(v);
+            if (h == v) {
+                strings_[h] = Set{};
+                heads[nh++] = h;
+            }
+        }
+    }
+    for (int v = W; v < W * (W - 1); v++) {
+        const int c = b.at(v);
+        if (c == BLACK || c == WHITE) set_bit(strings_[b.head(v)], v);
+    }
+    for (int i = 0; i < nh; i++) {
+        const Set n = neighbours(strings_[heads[i]], W);
+        Set &l = libs_[heads[i]];
+        for (int k = 0; k < kWords; k++) l.w[k] = n.w[k] & empty_.w[k];
+    }
+}
+
+int BoardBits::after_liberties_colour(int c, int v) const {
+    if (b_.is_suicide(v, c)) return 0;
+    int own[4], cap[4];
+    int nown = 0, ncap = 0;
+    for (int k = 0; k < 4; k++) {
+        const int a = v + b_.dir(k);
+        const int sc = b_.at(a);
+        if (sc != BLACK && sc != WHITE) continue;
+        const int h = b_.head(a);
+        int *list = sc == c ? own : cap;
+        int &cnt = sc == c ? nown : ncap;
+        if (sc != c && b_.string_liberties(a) != 1) continue;
+        bool dup = false;
+        for (int i = 0; i < cnt; i++) dup |= list[i] == h;
+        if (!dup) list[cnt++] = h;
+    }
+    if (ncap == 0) {
+        // Liberties of the merged string: the strings' liberties and v's
+        // empty neighbours, without v itself.
+        if (nown == 0) return b_.empty_neighbours(v);
+        Set l{};
+        for (int i = 0; i < nown; i++) or_into(l, libs_[own[i]]);
+        for (int k = 0; k < 4; k++) {
+            const int a = v + b_.dir(k);
+            if (b_.at(a) == EMPTY) set_bit(l, a);
+        }
+        l.w[v >> 6] &= ~(1ull << (v & 63));
+        int n = 0;
+        for (int i = 0; i < kWords; i++) n += __builtin_popcountll(l.w[i]);
+        return n;
+    }
+    Set m{}, e = empty_;
+    set_bit(m, v);
+    e.w[v >> 6] &= ~(1ull << (v & 63));
+    for (int i = 0; i < nown; i++) or_into(m, strings_[own[i]]);
+    for (int i = 0; i < ncap; i++) or_into(e, strings_[cap[i]]);
+    return count_and(neighbours(m, b_.width()), e);
+}
+
+std::pair<int, int> BoardBits::after_liberties(int c, int v) const {
+    return {after_liberties_colour(c, v), after_liberties_colour(other(c), v)};
+}
+
 // FastBoard::check_winning_ladder (FastBoard.cpp:2530-2564): an atari on a
 // neighbouring two-liberty enemy string at v whose escape reads as a
 // losing ladder for the enemy.

@@ -76,6 +76,11 @@ public:
 
     uint64_t stones_hash() const;  // FNV-1a over the squares (tests)
 
+    // Representative stone of the string through v, and the 4 neighbour
+    // offsets of the padded grid.
+    int head(int v) const { return head_[v]; }
+    int dir(int k) const { return dir_[k]; }
+
 private:
     int place(int c, int v, bool *capture, bool game_move);
     int remove_string(int v);
@@ -96,6 +101,36 @@ private:
     std::array<uint16_t, kMaxSquares> next_{};    // circular string list
     std::array<uint16_t, kMaxSquares + 1> libs_{};    // by representative
     std::array<uint16_t, kMaxSquares + 1> stones_{};  // by representative
+};
+
+// Bitboards of one board over padded vertex indices (441 bits in 7 words):
+// the empty points and every string's stones (indexed by representative).
+// They answer after_liberties without trial moves on board copies: after c
+// plays a legal, non-suicide v, the string through v is M = {v} plus the
+// adjacent c strings, the opponent strings whose only liberty was v are
+// captured (set C), and the string's liberties are the 4-neighbours of M
+// among the empty points E - {v} + C. Borders are never empty, so plain
+// shifts by 1 and by the grid width give the neighbours. Without a capture
+// that is the union of the merged strings' liberty sets and v's empty
+// neighbours, minus v.
+class BoardBits {
+public:
+    static constexpr int kWords = (kMaxSquares + 63) / 64;
+    struct Set {
+        uint64_t w[kWords];
+    };
+
+    explicit BoardBits(const Board &b);
+    // Same value as Board::after_liberties(c, v).
+    std::pair<int, int> after_liberties(int c, int v) const;
+
+private:
+    int after_liberties_colour(int c, int v) const;
+
+    const Board &b_;
+    Set empty_{};
+    std::array<Set, kMaxSquares> strings_;  // stones, by representative (live heads only)
+    std::array<Set, kMaxSquares> libs_;     // liberties, by representative (live heads only)
 };
 
 // FastState counterpart: the board plus move history, ko and komi.

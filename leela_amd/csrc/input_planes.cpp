@@ -32,6 +32,7 @@ void gather(const go::GameState &st, uint8_t *planes, const Layout &L) {
     std::memset(planes, 0, (size_t)kPlanes * P);
     const bool white_has_komi = !(std::fabs(st.komi) <= 0.75f);
     const int tomove = b.to_move();
+    const go::BoardBits bits(b);
     for (int y = 0; y < S; y++) {
         for (int x = 0; x < S; x++) {
             const int v = b.vertex(x, y);
@@ -50,7 +51,7 @@ void gather(const go::GameState &st, uint8_t *planes, const Layout &L) {
                         idx);
             } else {
                 set(planes, P, 0, idx);
-                const auto al = b.after_liberties(tomove, v);
+                const auto al = bits.after_liberties(tomove, v);
                 if (al.first >= 1)
                     set(planes, P, L.after_own + (al.first >= 6 ? 5 : al.first - 1),
                         idx);

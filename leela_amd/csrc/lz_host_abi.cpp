@@ -119,6 +119,37 @@ int lz_board_queries(const lz_position *pos, int32_t *out) {
     return LZ_OK;
 }
 
+int lz_encode_batch(const lz_position *const *pos, int n, int net,
+                    int symmetry, int threads, uint8_t *planes) {
+    if (n < 0 || (n > 0 && (!pos || !planes)) || symmetry < 0 || symmetry > 7 ||
+        (net != LZ_NET_POLICY && net != LZ_NET_VALUE))
+        return LZ_ERR_ARG;
+    if (n == 0) return LZ_OK;
+    for (int i = 0; i < n; i++)
+        if (!pos[i] || pos[i]->st.board.side() != pos[0]->st.board.side())
+            return LZ_ERR_ARG;
+    const int S = pos[0]->st.board.side();
+    const size_t per = (size_t)lz::kPlanes * S * S;
+    if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+    threads = std::max(1, std::min(threads, n));
+    auto work = [&](int t) {
+        std::vector<uint8_t> tmp(symmetry ? per : 0);
+        for (int i = t; i < n; i += threads) {
+            uint8_t *dst = symmetry ? tmp.data() : planes + per * i;
+            if (net == LZ_NET_POLICY)
+                lz::gather_features_policy(pos[i]->st, dst);
+            else
+                lz::gather_features_value(pos[i]->st, dst);
+            if (symmetry) lz::rotate_planes(tmp.data(), symmetry, S, planes + per * i);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    return LZ_OK;
+}
+
 int lz_make_batch(int n, int side, uint32_t seed0, int max_moves, int net,
                   int threads, uint8_t *planes) {
     if (n < 0 || !planes || side < 5 || side > lz::go::kMaxSide ||

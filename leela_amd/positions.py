@@ -42,6 +42,8 @@ def _lib():
         L.lz_make_batch.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_int,
                                     C.c_int, C.c_int, u8p]
         f32p = C.POINTER(C.c_float)
+        L.lz_encode_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_int,
+                                      C.c_int, C.c_int, u8p]
         L.lz_position_score.argtypes = [vp, f32p]
         L.lz_playout_winrate.argtypes = [vp, C.c_int, C.c_uint32, f32p]
         L.lz_make_value_labels.argtypes = [C.c_int, C.c_int, C.c_uint32,
@@ -135,6 +137,20 @@ def make_batch(n: int, seed0: int = 1, side: int = 19,
     _chk(_lib().lz_make_batch(n, side, seed0, max_moves, net, threads,
                               out.ctypes.data_as(C.POINTER(C.c_uint8))),
          "make_batch")
+    return out
+
+
+def encode_batch(positions, net: int = NET_POLICY, symmetry: int = 0,
+                 threads: int = 0) -> np.ndarray:
+    """Planes [n][32][S*S] of Position objects of one side, encoded on
+    `threads` host threads (lz_encode_batch)."""
+    n = len(positions)
+    side = positions[0].side if n else 19
+    out = np.zeros((n, 32, side * side), np.uint8)
+    arr = (C.c_void_p * max(n, 1))(*[p._p for p in positions])
+    _chk(_lib().lz_encode_batch(arr, n, net, symmetry, threads,
+                                out.ctypes.data_as(C.POINTER(C.c_uint8))),
+         "encode_batch")
     return out
 
 

@@ -236,3 +236,20 @@ def test_make_batch_matches_single_positions():
     for i in range(6):
         assert np.array_equal(b[i], Position.random(40 + i, 19, 250)
                               .planes(NET_VALUE))
+
+
+def test_encode_batch_matches_single_encoder():
+    """lz_encode_batch (threads, symmetries) gives the per-position encoder's
+    planes; the bitboard after-liberties inside it is pinned through the
+    plane tests above, which restate the planes from the reference's
+    per-point queries."""
+    ps = [positions.Position.random(s, 19, 250) for s in range(700, 760)]
+    for net in (positions.NET_POLICY, positions.NET_VALUE):
+        for sym in (0, 3, 6):
+            got = positions.encode_batch(ps, net, sym, threads=4)
+            ref = np.stack([p.planes(net, sym) for p in ps])
+            assert np.array_equal(got, ref), (net, sym)
+    small = [positions.Position.random(s, 9, 40) for s in range(5)]
+    assert positions.encode_batch(small, 0, 0, 2).shape == (5, 32, 81)
+    with pytest.raises(Exception):
+        positions.encode_batch(ps[:2] + small[:1], 0, 0, 1)
