@@ -295,8 +295,8 @@ int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
  * A leaf win-rate estimator trained on playout outcomes, standing in for
  * Playout::mc_owner (Playout.cpp:174-217) where a caller wants a playout
  * estimate; the reference has no learned counterpart. Network: 3x3 conv
- * 32->64 + relu, six residual blocks of two 3x3 64->64 convs, a 1x1 64->1
- * conv + ELU, FC 361->64 + ELU, FC 64->1, tanh; win rate (1 + t) / 2
+ * 32->64, six residual blocks of two 3x3 64->64 convs, a 1x1 64->1 conv,
+ * FC 361->64 (ELU after each), FC 64->1, tanh; win rate (1 + t) / 2
  * for the side to move, from the value planes (lz_encode_planes with
  * LZ_NET_VALUE). BF16 MFMA compute, FP32 master weights, Adam.
  * Parameters are a flat FP32 vector: per conv W[64][cin][3][3] then
@@ -339,7 +339,7 @@ int lz_surrogate_train_step_device(lz_surrogate *s, int n,
                                    float lr, float *loss, void *stream);
 /* Inspection of the last forward/backward pass, for layer-by-layer
  * checks. kind 0: the BF16 output map [n][361][64] of conv `layer` (0..12,
- * post-relu); kind 1: the BF16 gradient map at that conv's pre-activation,
+ * after ELU); kind 1: the BF16 gradient map at that conv's pre-activation,
  * available once lz_surrogate_keep_maps(s, 1) has been set (it keeps all
  * 13 maps instead of 3 rotating ones). Values are raw BF16 bit patterns.
  * lz_surrogate_read_head copies the head's saved s [n][361], f [n][64] and

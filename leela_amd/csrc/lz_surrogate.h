@@ -1,8 +1,8 @@
 // lz_surrogate.h — device kernels of the learned playout surrogate (not part
 // of the C ABI). The network:
 //
-//   a0 = relu(conv3x3(planes) + b0)                      32 -> 64
-//   block k = 1..6: u = relu(conv(h) + b), h' = relu(conv(u) + b + h)
+//   a0 = elu(conv3x3(planes) + b0)                       32 -> 64
+//   block k = 1..6: u = elu(conv(h) + b), h' = elu(conv(u) + b + h)
 //   s  = elu(wh . h6 + bh)    (1x1, 64 -> 1)
 //   f  = elu(F1 s + c1)       (361 -> 64)
 //   t  = tanh(F2 . f + c2),   win rate (1 + t) / 2
@@ -35,7 +35,7 @@ inline int conv_weights(int layer) { return kCh * conv_cin(layer) * 9; }
 hipError_t pack_conv(const float *w, uint16_t *wpk, int cin, bool bwd,
                      hipStream_t stream);
 
-// One conv over n boards. Forward (bwd = false): out = relu(conv(in) + bias
+// One conv over n boards. Forward (bwd = false): out = elu(conv(in) + bias
 // + res). Backward-data (bwd = true, cin 64): out = (conv(in) + res) masked
 // where mask > 0. res and mask may be null. in is u8 planes [n][32][361]
 // when cin == 32, else BF16 [n][361][64].

@@ -9,8 +9,9 @@
 //   is pre-packed in fragment order [k-step][m-tile][lane][8]: one 16-byte
 //   global load per lane (L1/L2 resident). Wave w owns pixel tiles w, w+4,
 //   ... (6 or 5 of the 23) for all 64 output channels. The FP32 epilogue
-//   adds bias and residual, applies relu (forward) or the mask of the
-//   forward activation (backward), and stores BF16 quads.
+//   adds bias and residual, applies ELU (forward) or multiplies by ELU's
+//   derivative taken from the stored forward activation (backward), and
+//   stores BF16 quads.
 //   Backward-data is the same kernel: dX = conv(dY; W') with
 //   W'[c][o][tap] = W[o][c][8 - tap] (taps flipped), packed by pack_conv.
 //
@@ -48,9 +49,10 @@ __device__ inline f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// The head's activation: ELU, as in the reference value head
-// (Network.cpp:676-740). A ReLU on the 1-channel plane can switch off for
-// every pixel and stop all learning; ELU keeps a gradient everywhere.
+// The activation of every layer: ELU, as in the reference nets
+// (Network.cpp:387-392). With ReLU, training runs collapsed to a constant
+// output (every unit of a layer switched off); ELU keeps a gradient
+// everywhere.
 __device__ inline float elu(float v) { return v > 0.f ? v : expf(v) - 1.f; }
 // d ELU / d v expressed through the output a = ELU(v).
 __device__ inline float elu_grad(float a) { return a > 0.f ? 1.f : a + 1.f; }
@@ -173,12 +175,12 @@ __global__ __launch_bounds__(kThreads, 2) void sg_conv(
             }
             if (!BWD) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) v[r] = fmaxf(v[r], 0.f);
+                for (int r = 0; r < 4; r++) v[r] = elu(v[r]);
             } else if (mask) {
                 const bf16x4 mv = *(const bf16x4 *)(mask + at);
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    if (!((float)mv[r] > 0.f)) v[r] = 0.f;
+                    v[r] *= elu_grad((float)mv[r]);
             }
             bf16x4 ov;
 #pragma unroll

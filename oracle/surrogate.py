@@ -4,10 +4,10 @@ The surrogate is this build's own network (the reference has none; its
 teacher is Playout::mc_owner, Playout.cpp:174-217), so the oracle restates
 the network's definition, not a reference routine:
 
-    a0  = relu(conv3x3(planes; W0) + b0)                      32 -> 64
+    a0  = elu(conv3x3(planes; W0) + b0)                       32 -> 64
     block k = 1..6:
-        u_k = relu(conv3x3(h_{k-1}; W_{2k-1}) + b_{2k-1})
-        h_k = relu(conv3x3(u_k; W_{2k}) + b_{2k} + h_{k-1})   (h_0 = a0)
+        u_k = elu(conv3x3(h_{k-1}; W_{2k-1}) + b_{2k-1})
+        h_k = elu(conv3x3(u_k; W_{2k}) + b_{2k} + h_{k-1})    (h_0 = a0)
     s   = elu(sum_c wh[c] h_6[c, p] + bh)                     1x1, 64 -> 1
     f   = elu(F1 s + c1)                                      361 -> 64
     t   = tanh(F2 . f + c2)                                   64 -> 1
@@ -162,7 +162,7 @@ def forward(flat, planes, bf16_points=False):
         res = h.reshape(n, CH, P) if (layer >= 2 and layer % 2 == 0) else None
         z, c = conv_pre(x, rnd(p[f"W{layer}"]), p[f"b{layer}"], res)
         cols.append(c)
-        a = rnd(np.maximum(z, 0.0)).reshape(n, CH, S, S)
+        a = rnd(_elu(z)).reshape(n, CH, S, S)
         acts.append(a)
         if layer == 0 or layer % 2 == 0:
             h = a
@@ -187,7 +187,7 @@ def backward(cache, y):
     g, ds = head_bwd(h6, s, f, t, y, p)
     # Map l: gradient at the pre-activation of conv l.
     maps = [None] * NCONV
-    maps[-1] = rnd(ds[:, None, :] * p["wh"][None, :, None] * (h6 > 0))
+    maps[-1] = rnd(ds[:, None, :] * p["wh"][None, :, None] * _elu_grad(h6))
     for layer in range(NCONV - 1, -1, -1):
         cin = 32 if layer == 0 else CH
         e = maps[layer]
@@ -199,7 +199,7 @@ def backward(cache, y):
             # First conv of a block: add the skip's gradient (map l + 1).
             dx = dx + maps[layer + 1]
         below = acts[layer - 1].reshape(n, CH, P)
-        maps[layer - 1] = rnd(dx * (below > 0))
+        maps[layer - 1] = rnd(dx * _elu_grad(below))
     cache["maps"] = maps
     flat = np.concatenate([g[name].ravel() for name, _, _ in layout()])
     return loss_of(t, y), flat

@@ -157,12 +157,12 @@ def test_kernels_layer_by_layer():
             return planes.reshape(n, 32, 19, 19).astype(np.float64)
         return A[layer - 1].reshape(n, 64, 19, 19)
 
-    # Forward convs: bias, skip and relu on the GPU's own input maps.
+    # Forward convs: bias, skip and ELU on the GPU's own input maps.
     for layer in range(orc.NCONV):
         res = A[layer - 2] if layer >= 2 and layer % 2 == 0 else None
         z, _ = orc.conv_pre(inp(layer), orc.bf16(p[f"W{layer}"]),
                             p[f"b{layer}"], res)
-        _same_bf16(A[layer], orc.bf16(np.maximum(z, 0.0)), f"conv {layer}")
+        _same_bf16(A[layer], orc.bf16(orc._elu(z)), f"conv {layer}")
     # Head forward and backward from the GPU's last map.
     s_o, f_o, t_o = orc.head_fwd(A[-1], p)
     np.testing.assert_allclose(sv, s_o, rtol=1e-4, atol=1e-5)
@@ -175,7 +175,7 @@ def test_kernels_layer_by_layer():
         np.testing.assert_allclose(grad(name), ref, rtol=1e-3,
                                    atol=1e-5 * np.abs(ref).max(), err_msg=name)
     _same_bf16(E[-1], orc.bf16(ds[:, None, :] * p["wh"][None, :, None]
-                               * (A[-1] > 0)), "head map")
+                               * orc._elu_grad(A[-1])), "head map")
     # Backward: weight gradients and gradient maps from the GPU's maps.
     for layer in range(orc.NCONV - 1, -1, -1):
         cin = 32 if layer == 0 else 64
@@ -191,7 +191,7 @@ def test_kernels_layer_by_layer():
         dx = orc.conv_bwd_data(E[layer], orc.bf16(p[f"W{layer}"]))
         if layer % 2 == 1:
             dx = dx + E[layer + 1]
-        _same_bf16(E[layer - 1], orc.bf16(dx * (A[layer - 1] > 0)),
+        _same_bf16(E[layer - 1], orc.bf16(dx * orc._elu_grad(A[layer - 1])),
                    f"map {layer - 1}")
 
 
