@@ -354,10 +354,30 @@ std::pair<int, int> BoardBits::after_liberties(int c, int v) const {
 // neighbouring two-liberty enemy string at v whose escape reads as a
 // losing ladder for the enemy.
 bool Board::check_winning_ladder(int c, int v) const {
+    // Whether the trial move captures: without a capture the escape's empty
+    // neighbours after it are known without playing it (below).
+    bool captures = false;
+    for (int k = 0; k < 4; k++) {
+        const int a = v + dir_[k];
+        captures |= sq_[a] == other(to_move_) && libs_[head_[a]] == 1;
+    }
     for (int k = 0; k < 4; k++) {
         const int a = v + dir_[k];
         if (sq_[a] != other(c) || libs_[head_[a]] != 2) continue;
         if (self_atari(c, v)) continue;
+        if (!captures) {
+            // The string keeps one liberty e (its other one); e's empty
+            // neighbours after the move lose v if adjacent. Only an escape
+            // with exactly two can start a ladder: skip the trial move
+            // otherwise (same answer, no board copy).
+            std::array<int, 2> lib{};
+            int cnt = 0;
+            collect_liberties<2>(a, lib, cnt);
+            const int e = lib[0] == v ? lib[1] : lib[0];
+            int en = empty_neighbours(e);
+            for (int j = 0; j < 4; j++) en -= e + dir_[j] == v;
+            if (en != 2) continue;
+        }
         Board t = *this;
         bool cap;
         t.play_fast(t.to_move_, v, &cap);
