@@ -84,8 +84,10 @@ int lz_forward(lz_ctx *ctx, int net, int precision, int n,
                const uint8_t *planes, float *raw, float *out);
 
 /* Same as lz_forward with device-resident buffers, enqueued on `stream`
- * (a hipStream_t, or NULL for the context's own stream); returns after
- * enqueueing. Buffers must stay valid until the stream reaches the work. */
+ * (a hipStream_t; NULL is the HIP default stream, as in every HIP call);
+ * returns after enqueueing. Buffers must stay valid until the stream
+ * reaches the work. Device-buffer calls on one context share its
+ * workspace: calls on different streams must be ordered by the caller. */
 int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
                       const uint8_t *d_planes, float *d_raw, float *d_out,
                       void *stream);
@@ -319,8 +321,8 @@ int lz_surrogate_set_adam(lz_surrogate *s, float beta1, float beta2,
                           float eps);
 long long lz_surrogate_step(const lz_surrogate *s);
 
-/* Win rates [n] of device planes [n][32][361] (u8), on `stream` (NULL:
- * the surrogate's own stream); asynchronous. */
+/* Win rates [n] of device planes [n][32][361] (u8), on `stream` (NULL: the
+ * HIP default stream); asynchronous. */
 int lz_surrogate_forward_device(lz_surrogate *s, int n, const uint8_t *planes,
                                 float *winrate, void *stream);
 /* Host planes in, host win rates out; synchronous. */

@@ -183,34 +183,6 @@ __device__ float block_max(float v, float *s_red) {
     return r;
 }
 
-// Pixel -> (column tile, lane) map of a 19x19 board. A B fragment read is
-// one ds_read_b128 per lane; 16 lanes of one k-group are served together
-// and, with the pixel stride PS = C + 8 halves (4 mod 64 banks in dwords),
-// they are conflict-free iff their window origins (21 y + x in the haloed
-// tile) differ mod 16. Raster tiles of 16 pixels break that at every row
-// end (the halo adds 2); this map puts the g-th pixel of residue class
-// r = (21 y + x) mod 16 in tile g, lane r (each class has at most 23
-// members, so 23 tiles hold the board; -1 marks padding).
-struct PixelMap {
-    short p[24][16];
-};
-
-constexpr PixelMap make_pixel_map() {
-    PixelMap m{};
-    int count[16] = {};
-    for (int g = 0; g < 24; g++)
-        for (int r = 0; r < 16; r++) m.p[g][r] = -1;
-    for (int y = 0; y < 19; y++)
-        for (int x = 0; x < 19; x++) {
-            const int r = (21 * y + x) % 16;
-            m.p[count[r]][r] = (short)(y * 19 + x);
-            count[r]++;
-        }
-    return m;
-}
-
-__constant__ PixelMap kPixelMap = make_pixel_map();
-
 // ---- FP8 e4m3 trunk ---------------------------------------------------------
 // Fragments of v_mfma_scale_f32_16x16x128_f8f6f4 (format 0 = e4m3, unit
 // block scales): A lane holds 32 k-values of row lane%16, B lane 32 k-values
@@ -516,13 +488,8 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++) {
-            int j;
-            if (RAGGED) {
-                j = (nt0 + t) * 16 + (lane & 15);
-                if (j >= npix) j = -1;
-            } else {
-                j = kPixelMap.p[nt0 + t][lane & 15];
-            }
+            int j = (nt0 + t) * 16 + (lane & 15);
+            if (j >= npix) j = -1;
             pix[t] = j;
             const int jj = j >= 0 ? j : 0;
             const int kb = jj / SS, r = jj % SS, y = r / S, x = r % S;

@@ -193,7 +193,7 @@ int lz_forward_ragged_device(lz_ctx *ctx, int net, int precision, int n,
     if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
         return ctx->fail(LZ_ERR_ARG, "bad net id %d", net);
     (void)hipSetDevice(ctx->device);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: default stream
     std::string msg;
     int rc = lz::build_ragged_layout(ctx->ragged, n, sides, s, &msg);
     if (rc == LZ_OK) rc = lz::ensure_workspace(ctx->ws, ctx->net, n, &msg);
@@ -232,7 +232,7 @@ int lz_forward_ragged(lz_ctx *ctx, int net, int precision, int n,
     int rc = e == hipSuccess ? LZ_OK : LZ_ERR_HIP;
     if (rc == LZ_OK)
         rc = lz_forward_ragged_device(ctx, net, precision, n, sides, d_planes,
-                                      d_raw, d_out, nullptr);
+                                      d_raw, d_out, ctx->stream);
     if (rc == LZ_OK && raw)
         e = hipMemcpyAsync(raw, d_raw, sizeof(float) * outs,
                            hipMemcpyDeviceToHost, ctx->stream);
@@ -257,7 +257,7 @@ int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
     if (!ctx) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: default stream
     lz::LaunchTimer timer;
     int rc = forward_locked(ctx, net, precision, n, d_planes, d_raw, d_out, s,
                             &timer);
@@ -283,7 +283,7 @@ int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
     if (!ctx) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: default stream
     return forward_locked(ctx, net, precision, n, d_planes, d_raw, d_out, s);
 }
 

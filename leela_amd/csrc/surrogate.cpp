@@ -319,7 +319,7 @@ int lz_surrogate_forward_device(lz_surrogate *s, int n, const uint8_t *planes,
     if (rc) return rc;
     if (!planes || !winrate) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(s->mu);
-    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
     rc = join(s, s->stream, st);
     if (!rc) rc = forward(s, n, planes, winrate, st);
     if (!rc) rc = join(s, st, s->stream);
@@ -370,7 +370,7 @@ int lz_surrogate_train_step_device(lz_surrogate *s, int n,
     if (rc) return rc;
     if (!planes || !targets || !(lr >= 0.f)) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(s->mu);
-    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
     rc = join(s, s->stream, st);
     if (!rc) rc = forward(s, n, planes, nullptr, st);
     if (!rc) rc = backward(s, n, planes, targets, st);

@@ -9,11 +9,12 @@ gradient maps agree to one BF16 rounding step (relative 2^-7) on all but
 0.1% of elements, FP32 gradients and head values to 1e-3 relative.
 
 End to end, rounding flips cascade through 13 BF16 layers (on the CPU, the
-oracle's own BF16 emulation moves by up to 6% per gradient tensor when only
-its accumulation precision changes), so the whole-network bars are: win
-rates within 1e-2 of the BF16-emulating oracle and 3e-2 of the exact one,
-per-tensor gradient error below 25% and whole-vector cosine >= 0.99
-against the BF16 oracle, >= 0.97 against the exact one."""
+oracle's own BF16 emulation moves by up to 0.5% per gradient tensor for
+these parameters when only its accumulation precision changes), so the
+whole-network bars are: win rates within 1e-2 of the BF16-emulating oracle
+and 3e-2 of the exact one, per-tensor gradient error below 5% and
+whole-vector cosine >= 0.999 against the BF16 oracle, >= 0.995 against the
+exact one."""
 import os
 
 import numpy as np
@@ -29,17 +30,20 @@ pytestmark = pytest.mark.gpu
 def _random_params(seed):
     """Every tensor drawn nonzero (the library starts the residual blocks'
     second convs and the output layer at zero, which would leave parts of
-    the gradient identically zero)."""
+    the gradient identically zero), scaled so that the ELU stack stays
+    moderate and tanh unsaturated (|t| < 0.5 on generator positions)."""
     rng = np.random.default_rng(seed)
     flat = np.zeros(orc.num_params(), np.float32)
     for name, shape, off in orc.layout():
         n = int(np.prod(shape))
         if name.startswith("W"):
-            std = np.sqrt(2.0 / (shape[1] * 9))
+            std = 0.7 * np.sqrt(2.0 / (shape[1] * 9))
         elif name == "F1":
             std = np.sqrt(2.0 / orc.P)
-        elif name in ("wh", "F2"):
-            std = np.sqrt(1.0 / 64)
+        elif name == "wh":
+            std = 0.125
+        elif name == "F2":
+            std = 0.02
         else:
             std = 0.05
         flat[off:off + n] = rng.normal(0, std, n)
@@ -101,13 +105,13 @@ def test_gradients_match_oracle(sur):
             assert np.linalg.norm(a) < 1e-6, name
             continue
         rel = np.linalg.norm(a - b) / nb
-        assert rel < 0.25, (name, rel)
+        assert rel < 0.05, (name, rel)
     g = g.astype(np.float64)
-    assert g @ go / np.linalg.norm(g) / np.linalg.norm(go) >= 0.99
+    assert g @ go / np.linalg.norm(g) / np.linalg.norm(go) >= 0.999
     t0, c0 = orc.forward(flat, planes)
     _, g0 = orc.backward(c0, y)
     cos = g @ g0 / np.linalg.norm(g) / np.linalg.norm(g0)
-    assert cos >= 0.97
+    assert cos >= 0.995
     # Same inputs, same gradient (fixed-order reductions).
     _, g2 = sur.gradients(planes, y)
     np.testing.assert_array_equal(g, g2)
