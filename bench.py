@@ -123,7 +123,12 @@ def main() -> int:
     d_pout = torch.empty((B, 361), dtype=torch.float32, device=dev)
     d_vraw = torch.empty((B, 1), dtype=torch.float32, device=dev)
     d_vout = torch.empty((B, 1), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream(dev).cuda_stream
+    # All device work of the benchmark runs on one non-default stream (the
+    # default stream's implicit synchronisation would sit inside the
+    # per-launch event timing).
+    bench_stream = torch.cuda.Stream(dev)
+    stream = bench_stream.cuda_stream
+    torch.cuda.synchronize(dev)
 
     def step(p, st=None):
         st = stream if st is None else st

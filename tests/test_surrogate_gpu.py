@@ -264,41 +264,44 @@ def test_batch_limits_rejected(sur):
 
 
 def test_training_beats_playouts_at_equal_cost():
-    """Gain over the method without a surrogate. Train on 4096 positions
-    labelled with 64 playouts each; on 512 held-out positions, measure MAE
-    against a fresh 256-playout mean. The surrogate must beat the constant
-    predictor (the training mean) and a 4-playout estimate, and answer far
-    faster than playouts do."""
+    """Gain over the method without a surrogate. Train on 16,384 generator
+    positions labelled with 32 playouts each (600 Adam steps of 256, lr
+    1e-3 after warm-up); on 1,024 held-out positions, measure the mean
+    absolute error against a fresh 256-playout mean. The surrogate must beat
+    the constant predictor by 20% and a 16-playout estimate (16x the
+    playouts the labels' noise would suggest), and answer faster than those
+    16 playouts run on the host threads."""
     import time
     import torch
-    print("surrogate training: labels 4096 x 64 playouts", flush=True)
+    print("surrogate training: labels 16384 x 32 playouts", flush=True)
     th = min(16, os.cpu_count() or 1)
-    tr_planes, tr_y = positions.value_labels(4096, 30000, 64, 3_000_000,
+    tr_planes, tr_y = positions.value_labels(16384, 30000, 32, 3_000_000,
                                              threads=th)
-    ev_planes, ev_y = positions.value_labels(512, 40000, 256, 4_000_000,
+    ev_planes, ev_y = positions.value_labels(1024, 40000, 256, 4_000_000,
                                              threads=th)
-    s = Surrogate(0, max_batch=512, seed=7)
+    s = Surrogate(0, max_batch=1024, seed=7)
     try:
         print("surrogate training: 600 steps, batch 256", flush=True)
         losses = train(s, tr_planes, tr_y, steps=600, batch=256, lr=1e-3,
                        seed=1, log_every=100)
-        assert losses[-1][1] < 0.7 * losses[0][1]
+        assert losses[-1][1] < 0.8 * losses[0][1]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pred = s.predict(ev_planes)
         t_sur = time.perf_counter() - t0
     finally:
         s.close()
+    t0 = time.perf_counter()
+    _, est16 = positions.value_labels(1024, 40000, 16, 5_000_000,
+                                      threads=th, planes=False)
+    t_po = time.perf_counter() - t0
     mae_sur = float(np.mean(np.abs(pred - ev_y)))
     mae_const = float(np.mean(np.abs(tr_y.mean() - ev_y)))
-    ps = [positions.Position.random(40000 + i, 19, 250) for i in range(512)]
-    t0 = time.perf_counter()
-    est4 = LeafEvaluator(None, playouts=4, seed=5_000_000).winrates(ps)
-    t_po = time.perf_counter() - t0
-    mae_4 = float(np.mean(np.abs(est4 - ev_y)))
+    mae_16 = float(np.mean(np.abs(est16 - ev_y)))
     print(f"MAE surrogate {mae_sur:.4f} constant {mae_const:.4f} "
-          f"4-playout {mae_4:.4f}; time surrogate {t_sur * 1e3:.1f} ms, "
-          f"4 playouts {t_po * 1e3:.1f} ms", flush=True)
-    assert mae_sur < mae_const
-    assert mae_sur < mae_4
+          f"16-playout {mae_16:.4f}; time surrogate {t_sur * 1e3:.1f} ms, "
+          f"16 playouts x 1024 positions on {th} threads {t_po * 1e3:.1f} ms",
+          flush=True)
+    assert mae_sur < 0.8 * mae_const
+    assert mae_sur < mae_16
     assert t_sur < t_po
