@@ -137,8 +137,19 @@ def main() -> int:
         be.forward_device(leela_amd.NET_VALUE, p, B, d_vplanes.data_ptr(),
                           d_vraw.data_ptr(), d_vout.data_ptr(), st)
 
+    def settle(p, seconds=0.3):
+        # Back-to-back steps until the device has been busy for `seconds`:
+        # after an idle gap (host-side work) the first launches run at a
+        # lower clock, which would otherwise leak into the timings below.
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            for _ in range(4):
+                step(p)
+            torch.cuda.synchronize(dev)
+
     def run(name, steps, warmup):
         p = leela_amd.PRECISIONS[name]
+        settle(p)
         for _ in range(warmup):
             step(p)
         torch.cuda.synchronize(dev)
@@ -234,8 +245,7 @@ def main() -> int:
         print(f"graph replay skipped: {err}", file=sys.stderr)
 
     # Host feature encoder (both nets' planes from the batch's positions,
-    # lz_encode_batch) on the host threads: the end-to-end rate is bounded
-    # by the slower of the encoder and the device when the two overlap.
+    # lz_encode_batch) on the host threads.
     enc_threads = min(16, os.cpu_count() or 1)
     pos_objs = [positions.Position.random(1 + i) for i in range(B)]
     positions.encode_batch(pos_objs[:64], leela_amd.NET_POLICY, 0, enc_threads)
@@ -245,9 +255,52 @@ def main() -> int:
         positions.encode_batch(pos_objs, leela_amd.NET_VALUE, 0, enc_threads)
     enc_pos_s = 3 * B / (time.perf_counter() - t0)
 
+    # Device encoder (lz_encode_device): both nets' planes computed on the
+    # GPU from the positions (host side: one record per board, one copy).
+    # End to end: encode both nets' planes and run both nets, on the
+    # benchmark stream, positions in host memory at the start of each step.
+    denc = positions.DeviceEncoder(0, max_batch=B)
+    d_epl = torch.empty_like(d_planes)
+    d_evpl = torch.empty_like(d_vplanes)
+
+    def encode_step():
+        denc.encode_into(pos_objs, d_epl.data_ptr(), leela_amd.NET_POLICY, 0,
+                         stream)
+        denc.encode_into(pos_objs, d_evpl.data_ptr(), leela_amd.NET_VALUE, 0,
+                         stream)
+
+    def e2e_step():
+        encode_step()
+        be.forward_device(leela_amd.NET_POLICY, p, B, d_epl.data_ptr(),
+                          d_praw.data_ptr(), d_pout.data_ptr(), stream)
+        be.forward_device(leela_amd.NET_VALUE, p, B, d_evpl.data_ptr(),
+                          d_vraw.data_ptr(), d_vout.data_ptr(), stream)
+
+    encode_step()
+    torch.cuda.synchronize(dev)
+    denc_exact = bool(torch.equal(d_epl, d_planes) and
+                      torch.equal(d_evpl, d_vplanes))
+    reps = max(3, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        encode_step()
+    torch.cuda.synchronize(dev)
+    denc_pos_s = reps * B / (time.perf_counter() - t0)
+    settle(p)
+    for _ in range(2):
+        e2e_step()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e2e_step()
+    torch.cuda.synchronize(dev)
+    e2e_pos_s = args.steps * B / (time.perf_counter() - t0)
+    denc.close()
+
     # Dominant kernel, timed per launch with HIP events on its stream.
     kern_ms = []
-    for _ in range(5):
+    settle(p)
+    for _ in range(20):
         ms = be.forward_timed(leela_amd.NET_POLICY, p, B, d_planes.data_ptr(),
                               d_praw.data_ptr(), d_pout.data_ptr(), stream)
         kern_ms += list(ms[2:12]) if head == "fp32" else list(ms)
@@ -302,9 +355,14 @@ def main() -> int:
             "host_buffer_positions_per_s": round(host_pos_s, 1),
             "graph_replay_positions_per_s": (round(graph_pos_s, 1)
                                              if graph_pos_s else None),
-            "encoder_positions_per_s": round(enc_pos_s, 1),
-            "encoder_threads": enc_threads,
-            "end_to_end_positions_per_s": round(min(enc_pos_s, value), 1),
+            "host_encoder_positions_per_s": round(enc_pos_s, 1),
+            "host_encoder_threads": enc_threads,
+            "device_encoder_positions_per_s": round(denc_pos_s, 1),
+            "device_encoder_planes_equal_host": denc_exact,
+            "end_to_end_positions_per_s": round(e2e_pos_s, 1),
+            "end_to_end": "positions in host memory -> lz_encode_device "
+                          "(both nets) -> policy + value forward, one "
+                          "stream, one host thread",
         },
         "roofline": {
             "bound": "mfma",

@@ -235,6 +235,31 @@ int lz_board_queries(const lz_position *pos, int32_t *out);
 int lz_encode_batch(const lz_position *const *pos, int n, int net,
                     int symmetry, int threads, uint8_t *planes);
 
+/* ---- Device feature encoder -------------------------------------------
+ * The same planes computed on the GPU, one wavefront per board, written
+ * straight into device memory where the nets read them: the per-leaf
+ * gather_features_policy / gather_features_value calls (Network.cpp:557,
+ * 597, 633; encoders at :883-1201) moved off the host. The host side only
+ * copies each board's squares and string representatives into a record
+ * (1408 bytes) and sends the batch in one copy. */
+typedef struct lz_encoder lz_encoder;
+
+/* Pinned staging and device records for batches up to max_batch boards on
+ * `device`. */
+int lz_encoder_create(lz_encoder **out, int device, int max_batch);
+void lz_encoder_destroy(lz_encoder *enc);
+const char *lz_encoder_last_error(const lz_encoder *enc);
+
+/* Planes of n positions (sides may differ) into device memory d_planes,
+ * packed board after board, uint8 [32][S_i*S_i] each (the layout of
+ * lz_forward_device for one side, of lz_forward_ragged_device for mixed
+ * sides), each equal to lz_encode_planes(pos[i], net, symmetry).
+ * Enqueued on `stream` (NULL: the HIP default stream); returns once the
+ * positions have been read, so they may change afterwards. Calls on one
+ * encoder on different streams must be ordered by the caller. */
+int lz_encode_device(lz_encoder *enc, const lz_position *const *pos, int n,
+                     int net, int symmetry, uint8_t *d_planes, void *stream);
+
 /* Generator + encoder for n positions with seeds seed0..seed0+n-1:
  * planes uint8 [n][32][side*side] of `net`, rotation 0. threads <= 0 uses
  * every hardware thread. */

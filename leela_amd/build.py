@@ -39,7 +39,7 @@ def hipcc() -> str:
 # Sources with no device code and no HIP runtime calls (the board engine,
 # encoders, playouts and their C ABI): built by the host C++ compiler.
 HOST_ONLY = {"go_board.cpp", "input_planes.cpp", "playout.cpp",
-             "lz_host_abi.cpp"}
+             "lz_host_abi.cpp", "lz_enc_record.cpp"}
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-mpopcnt", "-Wall",
               f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 

@@ -381,8 +381,11 @@ bool Board::check_winning_ladder(int c, int v) const {
         Board t = *this;
         bool cap;
         t.play_fast(t.to_move_, v, &cap);
+        // A trial move that captures next to the string can leave it more
+        // than one liberty: no escape point (vertex 0 has no empty
+        // neighbours in the reference's neighbour counts).
         const int escape = t.atari_liberty(a);
-        if (t.empty_neighbours(escape) == 2) {
+        if (escape && t.empty_neighbours(escape) == 2) {
             t.set_to_move(other(t.to_move_));
             if (t.check_losing_ladder(t.to_move_, escape)) return true;
         }

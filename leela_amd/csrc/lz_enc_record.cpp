@@ -1,0 +1,29 @@
+// lz_enc_record.cpp — packing a game state into a device encoder record.
+#include "lz_enc_record.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace lz {
+
+void pack_record(const go::GameState &st, int64_t offset, EncRecord *r) {
+    const go::Board &b = st.board;
+    const int W = b.width(), G = W * W;
+    for (int v = 0; v < G; v++) {
+        const int c = b.at(v);
+        r->sq[v] = (uint8_t)c;
+        r->head[v] = (c == go::BLACK || c == go::WHITE) ? (uint16_t)b.head(v)
+                                                        : (uint16_t)0;
+    }
+    std::memset(r->sq + G, go::OFFBOARD, kEncGrid - G);
+    std::memset(r->head + G, 0, sizeof(uint16_t) * (kEncGrid - G));
+    r->side = b.side();
+    r->to_move = b.to_move();
+    r->komove = st.komove;
+    r->last_move = st.last_move;
+    r->prev_last = st.prev_last;
+    r->white_has_komi = !(std::fabs(st.komi) <= 0.75f);
+    r->out_offset = offset;
+}
+
+}  // namespace lz

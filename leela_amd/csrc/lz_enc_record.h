@@ -1,0 +1,31 @@
+// lz_enc_record.h — one board as the device feature encoder reads it.
+#pragma once
+
+#include <cstdint>
+
+#include "go_board.h"
+
+namespace lz {
+
+// The grid is the padded (side+2)^2 board of the reference (FastBoard
+// vertex numbering), row-major.
+constexpr int kEncGrid = 448;  // >= 21 * 21, a multiple of 64
+
+struct alignas(64) EncRecord {
+    uint8_t sq[kEncGrid];      // 0 black, 1 white, 2 empty, 3 off-board
+    uint16_t head[kEncGrid];   // string representative of each stone
+    int32_t side;              // 5..19
+    int32_t to_move;           // 0 black, 1 white
+    int32_t komove;            // simple-ko vertex, <= 0 when none
+    int32_t last_move;         // vertex, <= 0 when none or a pass
+    int32_t prev_last;         // vertex, <= 0 when none or a pass
+    int32_t white_has_komi;    // |komi| > 0.75
+    int64_t out_offset;        // byte offset of this board's planes
+    uint8_t pad_[32];
+};
+static_assert(sizeof(EncRecord) == 1408, "record layout");
+
+// Fills r from a game state; the board's planes go to out + offset.
+void pack_record(const go::GameState &st, int64_t offset, EncRecord *r);
+
+}  // namespace lz

@@ -49,6 +49,13 @@ def _lib():
         L.lz_make_value_labels.argtypes = [C.c_int, C.c_int, C.c_uint32,
                                            C.c_int, C.c_int, C.c_uint32,
                                            C.c_int, u8p, f32p]
+        L.lz_encoder_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
+        L.lz_encoder_destroy.argtypes = [vp]
+        L.lz_encoder_destroy.restype = None
+        L.lz_encoder_last_error.argtypes = [vp]
+        L.lz_encoder_last_error.restype = C.c_char_p
+        L.lz_encode_device.argtypes = [vp, C.POINTER(vp), C.c_int, C.c_int,
+                                       C.c_int, vp, vp]
         L._positions_bound = True
     return L
 
@@ -152,6 +159,62 @@ def encode_batch(positions, net: int = NET_POLICY, symmetry: int = 0,
                                 out.ctypes.data_as(C.POINTER(C.c_uint8))),
          "encode_batch")
     return out
+
+
+class DeviceEncoder:
+    """The feature encoder on the GPU (lz_encode_device): planes of a batch
+    of Position objects written into device memory, one wavefront per
+    board. Sides may differ within a batch (planes packed board after
+    board)."""
+
+    def __init__(self, device: int = 0, max_batch: int = 1024):
+        self._L = _lib()
+        h = C.c_void_p()
+        _chk(self._L.lz_encoder_create(C.byref(h), device, max_batch),
+             "lz_encoder_create")
+        self._h = h
+        self.device = device
+        self.max_batch = max_batch
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lz_encoder_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def encode_into(self, positions, d_planes: int, net: int = NET_POLICY,
+                    symmetry: int = 0, stream: int = 0):
+        """Enqueue the planes of `positions` into device address d_planes
+        on `stream` (0: the HIP default stream)."""
+        n = len(positions)
+        arr = (C.c_void_p * max(n, 1))(*[p._p for p in positions])
+        rc = self._L.lz_encode_device(self._h, arr, n, net, symmetry,
+                                      C.c_void_p(d_planes),
+                                      C.c_void_p(stream))
+        if rc != LZ_OK:
+            raise LeelaError(rc, self._L.lz_encoder_last_error(self._h)
+                             .decode())
+
+    def encode(self, positions, net: int = NET_POLICY, symmetry: int = 0):
+        """Planes as a uint8 CUDA tensor: [n, 32, S*S] for one side, flat
+        for mixed sides."""
+        import torch
+        sides = {p.side for p in positions}
+        total = sum(32 * p.side * p.side for p in positions)
+        dev = torch.device("cuda", self.device)
+        out = torch.empty(max(total, 1), dtype=torch.uint8, device=dev)
+        self.encode_into(positions, out.data_ptr(), net, symmetry,
+                         torch.cuda.current_stream(dev).cuda_stream)
+        out = out[:total]
+        if len(sides) == 1:
+            side = sides.pop()
+            out = out.view(len(positions), 32, side * side)
+        return out
 
 
 def value_labels(n: int, seed0: int, playouts: int, playout_seed: int,

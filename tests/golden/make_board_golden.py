@@ -5,7 +5,8 @@ sources by oracle/ref_board/Makefile).
 
 Cases are (side, seed, max_moves, prefix): the synthetic-position generator
 is run on the reference engine from `seed`; prefix -1 keeps the whole game,
-otherwise the position after the first `prefix` moves of that game. Full
+otherwise the position after the first `prefix` moves of that game;
+prefix -2 marks a crafted ladder-rich position (crafted_moves). Full
 games cover 19x19, 13x13 and 9x9; prefixes are picked to include simple-ko
 states and losing-ladder escapes, which random-play end positions rarely
 contain. Each case stores the moves, the FastState fields, the squares and
@@ -43,6 +44,50 @@ def pick_prefixes(n_ko=12, n_ladder=24, seeds=range(1, 400)):
     return ko + lad
 
 
+def crafted_moves(seed, side=19):
+    """Move list (passes give the colours) of a ladder-rich position: 1-4
+    ladder starters (a string of the side to move in atari with its escape
+    open, or an opponent stone with two liberties), then random stones of
+    both colours at density 3-20% as breakers and ladder makers. Moves are
+    chosen with the library's board (empty, not suicide); the expected
+    values come from replaying them on the reference engine."""
+    from leela_amd.positions import Position
+    rng = np.random.default_rng(seed)
+    p = Position(side)
+    moves = []
+
+    def place(c, x, y):
+        if p.squares()[y * side + x] != 2:
+            return
+        if p.info()["to_move"] != c:
+            p.play(-1)
+            moves.append(-1)
+        if p.queries()[y * side + x][9] == 1:
+            return
+        v = p.vertex(x, y)
+        p.play(v)
+        moves.append(v)
+
+    tm = int(rng.integers(0, 2))
+    for _ in range(rng.integers(1, 5)):
+        x, y = (int(t) for t in rng.integers(2, side - 2, 2))
+        dx, dy = (int(t) for t in rng.choice([-1, 1], 2))
+        d = tm if rng.random() < 0.5 else 1 - tm
+        place(d, x, y)
+        place(1 - d, x + dx, y)
+        place(1 - d, x, y + dy)
+        if d == tm:
+            place(1 - d, x - dx, y)
+    for _ in range(int(rng.uniform(0.03, 0.2) * side * side)):
+        c = int(rng.integers(0, 2))
+        x, y = (int(t) for t in rng.integers(0, side, 2))
+        place(c, x, y)
+    if p.info()["to_move"] != tm:
+        p.play(-1)
+        moves.append(-1)
+    return np.array(moves, np.int32)
+
+
 def main():
     if not rb.build():
         raise SystemExit("reference sources not found; cannot build checker")
@@ -50,9 +95,18 @@ def main():
              + [(19, s, 200, -1) for s in range(1000, 1008)]
              + [(13, s, 101, -1) for s in range(1, 25)]
              + [(9, s, 48, -1) for s in range(1, 25)]
-             + pick_prefixes())
+             + pick_prefixes()
+             + [(19, s, 0, -2) for s in range(1, 61)])
     out = {"cases": np.array(cases, np.int32)}
     for i, (side, seed, mm, prefix) in enumerate(cases):
+        if prefix == -2:
+            moves = crafted_moves(seed, side)
+            info, sq, q = rb.replay(side, 7.5, moves)
+            out[f"info_{i}"] = info
+            out[f"squares_{i}"] = sq
+            out[f"queries_{i}"] = q
+            out[f"moves_{i}"] = moves
+            continue
         info, sq, q, moves = rb.random_position(side, seed, mm)
         if prefix >= 0:
             moves = moves[:prefix]

@@ -62,7 +62,7 @@ def test_replayed_positions_match_reference(gold):
 def test_generator_reproduces_reference_games(gold):
     cases = gold["cases"]
     for i, (side, seed, mm, prefix) in enumerate(cases):
-        if prefix >= 0:
+        if prefix != -1:  # prefixes and crafted positions
             continue
         p = Position.random(int(seed), int(side), int(mm))
         assert np.array_equal(info_vec(p), gold[f"info_{i}"]), (i, seed)

@@ -1,0 +1,128 @@
+"""The device feature encoder's kernel source (leela_amd/csrc/encoder.hip)
+run on the CPU, before any GPU run, against the reference's own board
+queries and the host encoder.
+
+The kernel is compiled by g++ against tests/wave_emul/hip/hip_runtime.h,
+which runs each workgroup as 64 host threads (one wavefront): __shared__
+data shared by them, __syncthreads a barrier, __ballot a gather of the 64
+predicates. The same source, unchanged, is what hipcc builds for gfx950;
+the GPU tests compare that build with the same expectations.
+
+Pins: the planes restated from tests/golden/board_golden.npz (queries of
+the reference's FastBoard, including crafted ladder-rich positions), the
+host encoder on generator positions of mixed sides, and the symmetry
+identity planes(s)[i] = planes(0)[rotate(i, s)].
+"""
+import ctypes as C
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from leela_amd.positions import NET_POLICY, NET_VALUE
+
+from test_board_cpu import expected_planes, rot_closed_form
+
+ROOT = Path(__file__).resolve().parents[1]
+EMUL = Path(__file__).parent / "wave_emul"
+CSRC = ROOT / "leela_amd" / "csrc"
+OUT = ROOT / "_build" / "wave_emul" / "libencoder_emul.so"
+GOLD = Path(__file__).parent / "golden" / "board_golden.npz"
+SOURCES = [CSRC / "encoder.hip", EMUL / "driver.cpp", CSRC / "go_board.cpp",
+           CSRC / "input_planes.cpp", CSRC / "lz_enc_record.cpp"]
+
+
+def _build():
+    deps = SOURCES + [EMUL / "hip" / "hip_runtime.h"] + list(CSRC.glob("*.h"))
+    if OUT.exists() and OUT.stat().st_mtime > max(p.stat().st_mtime
+                                                   for p in deps):
+        return OUT
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    OUT.parent.mkdir(parents=True, exist_ok=True)
+    cmd = [cxx, "-std=c++20", "-O2", "-pthread", "-fPIC", "-shared",
+           "-Wno-unknown-pragmas", f"-I{EMUL}", f"-I{CSRC}",
+           f"-I{ROOT / 'include'}", "-x", "c++", str(SOURCES[0]), "-x",
+           "none", *map(str, SOURCES[1:]), "-o", str(OUT)]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return OUT
+
+
+@pytest.fixture(scope="module")
+def emul():
+    L = C.CDLL(str(_build()))
+    i32p = C.POINTER(C.c_int32)
+    u8p = C.POINTER(C.c_uint8)
+    L.emul_encode_moves.argtypes = [C.c_int, C.c_int, i32p, C.c_int, C.c_int,
+                                    u8p]
+    L.emul_encode_generated.argtypes = [C.c_int, i32p,
+                                        C.POINTER(C.c_uint32), i32p, C.c_int,
+                                        C.c_int, u8p, u8p]
+    return L
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def encode_moves(L, side, moves, net, sym=0):
+    out = np.full((32, side * side), 0xAB, np.uint8)
+    mv = np.ascontiguousarray(moves, np.int32)
+    L.emul_encode_moves(side, len(mv), _p(mv, C.c_int32), net, sym,
+                        _p(out, C.c_uint8))
+    return out
+
+
+@pytest.mark.parametrize("net", [NET_POLICY, NET_VALUE])
+def test_kernel_matches_reference_queries(emul, gold, net):
+    """Every golden board (random games, ko and ladder prefixes, crafted
+    ladder-rich positions): the kernel's planes equal the layout restated
+    from the reference's queries."""
+    for i, (side, *_rest) in enumerate(gold["cases"]):
+        got = encode_moves(emul, int(side), gold[f"moves_{i}"], net)
+        want = expected_planes(net, int(side), gold[f"info_{i}"],
+                               gold[f"squares_{i}"], gold[f"queries_{i}"])
+        assert np.array_equal(got, want), (i, np.argwhere(got != want)[:5])
+
+
+def test_kernel_symmetries(emul, gold):
+    cases = gold["cases"]
+    for i in (0, 80, 100, len(cases) - 1):
+        side = int(cases[i][0])
+        base = encode_moves(emul, side, gold[f"moves_{i}"], NET_POLICY)
+        for s in range(1, 8):
+            got = encode_moves(emul, side, gold[f"moves_{i}"], NET_POLICY, s)
+            idx = [rot_closed_form(k, s, side) for k in range(side * side)]
+            assert np.array_equal(got, base[:, idx]), (i, s)
+
+
+@pytest.mark.parametrize("net,sym", [(NET_POLICY, 0), (NET_POLICY, 5),
+                                     (NET_VALUE, 3)])
+def test_kernel_matches_host_encoder_ragged(emul, net, sym):
+    """A ragged batch of generator positions (19/13/9, move caps 250 / 101
+    / 48): one launch, planes packed board after board, equal to the host
+    encoder board by board."""
+    n = 150
+    sides = np.array([(19, 13, 9)[i % 3] for i in range(n)], np.int32)
+    seeds = np.arange(5000, 5000 + n, dtype=np.uint32)
+    mm = np.array([250 if s == 19 else int(0.6 * s * s) for s in sides],
+                  np.int32)
+    tot = int(sum(32 * s * s for s in sides))
+    out = np.full(tot, 0xAB, np.uint8)
+    ref = np.zeros(tot, np.uint8)
+    emul.emul_encode_generated(n, _p(sides, C.c_int32), _p(seeds, C.c_uint32),
+                               _p(mm, C.c_int32), net, sym,
+                               _p(out, C.c_uint8), _p(ref, C.c_uint8))
+    off = 0
+    for i, s in enumerate(sides):
+        sz = 32 * int(s) * int(s)
+        assert np.array_equal(out[off:off + sz], ref[off:off + sz]), (i, s)
+        off += sz
