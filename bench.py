@@ -255,31 +255,41 @@ def main() -> int:
         positions.encode_batch(pos_objs, leela_amd.NET_VALUE, 0, enc_threads)
     enc_pos_s = 3 * B / (time.perf_counter() - t0)
 
-    # Device encoder (lz_encode_device): both nets' planes computed on the
-    # GPU from the positions (host side: one record per board, one copy).
-    # End to end: encode both nets' planes and run both nets, on the
-    # benchmark stream, positions in host memory at the start of each step.
+    # Device encoder (lz_encode_device_both): both nets' planes computed on
+    # the GPU from the positions (host side: one record per board, one
+    # copy). End to end: positions in host memory -> planes -> policy +
+    # value outputs, the encoder on its own stream one batch ahead of the
+    # nets (two plane buffers, events between the streams), as a search
+    # would feed it.
     denc = positions.DeviceEncoder(0, max_batch=B)
-    d_epl = torch.empty_like(d_planes)
-    d_evpl = torch.empty_like(d_vplanes)
+    pbatch = positions.PositionBatch(pos_objs)
+    d_epl = [torch.empty_like(d_planes) for _ in range(2)]
+    d_evpl = [torch.empty_like(d_vplanes) for _ in range(2)]
+    enc_stream = torch.cuda.Stream(dev)
+    nn_stream = bench_stream
+    ev_enc = [torch.cuda.Event() for _ in range(2)]
+    ev_nn = [torch.cuda.Event() for _ in range(2)]
 
-    def encode_step():
-        denc.encode_into(pos_objs, d_epl.data_ptr(), leela_amd.NET_POLICY, 0,
-                         stream)
-        denc.encode_into(pos_objs, d_evpl.data_ptr(), leela_amd.NET_VALUE, 0,
-                         stream)
+    def encode_step(j=0, st=stream):
+        denc.encode_both_into(pbatch, d_epl[j].data_ptr(),
+                              d_evpl[j].data_ptr(), 0, st)
 
-    def e2e_step():
-        encode_step()
-        be.forward_device(leela_amd.NET_POLICY, p, B, d_epl.data_ptr(),
+    def e2e_step(k):
+        j = k % 2
+        enc_stream.wait_event(ev_nn[j])  # the nets are done with buffer j
+        encode_step(j, enc_stream.cuda_stream)
+        ev_enc[j].record(enc_stream)
+        nn_stream.wait_event(ev_enc[j])
+        be.forward_device(leela_amd.NET_POLICY, p, B, d_epl[j].data_ptr(),
                           d_praw.data_ptr(), d_pout.data_ptr(), stream)
-        be.forward_device(leela_amd.NET_VALUE, p, B, d_evpl.data_ptr(),
+        be.forward_device(leela_amd.NET_VALUE, p, B, d_evpl[j].data_ptr(),
                           d_vraw.data_ptr(), d_vout.data_ptr(), stream)
+        ev_nn[j].record(nn_stream)
 
     encode_step()
     torch.cuda.synchronize(dev)
-    denc_exact = bool(torch.equal(d_epl, d_planes) and
-                      torch.equal(d_evpl, d_vplanes))
+    denc_exact = bool(torch.equal(d_epl[0], d_planes) and
+                      torch.equal(d_evpl[0], d_vplanes))
     reps = max(3, args.steps // 2)
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -287,12 +297,12 @@ def main() -> int:
     torch.cuda.synchronize(dev)
     denc_pos_s = reps * B / (time.perf_counter() - t0)
     settle(p)
-    for _ in range(2):
-        e2e_step()
+    for k in range(2):
+        e2e_step(k)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e2e_step()
+    for k in range(args.steps):
+        e2e_step(k)
     torch.cuda.synchronize(dev)
     e2e_pos_s = args.steps * B / (time.perf_counter() - t0)
     denc.close()
@@ -360,9 +370,10 @@ def main() -> int:
             "device_encoder_positions_per_s": round(denc_pos_s, 1),
             "device_encoder_planes_equal_host": denc_exact,
             "end_to_end_positions_per_s": round(e2e_pos_s, 1),
-            "end_to_end": "positions in host memory -> lz_encode_device "
-                          "(both nets) -> policy + value forward, one "
-                          "stream, one host thread",
+            "end_to_end": "positions in host memory -> "
+                          "lz_encode_device_both (own stream, one batch "
+                          "ahead) -> policy + value forward, one host "
+                          "thread",
         },
         "roofline": {
             "bound": "mfma",

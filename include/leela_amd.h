@@ -260,6 +260,14 @@ const char *lz_encoder_last_error(const lz_encoder *enc);
 int lz_encode_device(lz_encoder *enc, const lz_position *const *pos, int n,
                      int net, int symmetry, uint8_t *d_planes, void *stream);
 
+/* Both nets' planes from one pass over the boards (the string, liberty
+ * and ladder reading is shared): d_policy as lz_encode_device with
+ * LZ_NET_POLICY, d_value as with LZ_NET_VALUE. get_scored_moves and
+ * get_value on the same leaf (Network.cpp:557, 597) need both. */
+int lz_encode_device_both(lz_encoder *enc, const lz_position *const *pos,
+                          int n, int symmetry, uint8_t *d_policy,
+                          uint8_t *d_value, void *stream);
+
 /* Generator + encoder for n positions with seeds seed0..seed0+n-1:
  * planes uint8 [n][32][side*side] of `net`, rotation 0. threads <= 0 uses
  * every hardware thread. */

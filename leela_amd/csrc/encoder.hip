@@ -17,9 +17,13 @@
 //   3. lane per empty point: after-move liberties of both colours (the
 //      merged string's neighbours among the empty points plus captured
 //      stones) and the ladder candidates, queued as tasks;
-//   4. lane per task: the ladder readers on a private bitboard board,
-//      branching through a small stack in private memory;
-//   5. lane per output index: the 32 plane bytes of the rotated point.
+//   4. a group of 8 lanes per ladder task: the readers on their own copy
+//      of the board, one bitboard word per lane, branching through a small
+//      stack in private memory; string fills start from the board's own
+//      strings (LDS), so a fill costs the stones the reading added, not
+//      the string's length;
+//   5. lane per output index: the 32 plane bytes of the rotated point, for
+//      the policy net's layout, the value net's, or both from one pass.
 // Every output byte is written, so the output needs no clearing.
 #include <hip/hip_runtime.h>
 
@@ -30,8 +34,6 @@ namespace {
 
 constexpr int kWords = 7;
 constexpr int kBlack = 0, kWhite = 1, kEmpty = 2, kOff = 3;
-constexpr int kMaxPoints = 361;
-constexpr int kMaxTasks = 5 * kMaxPoints;  // <= 4 winning + 1 losing per point
 constexpr int kMaxBranch = 5;              // FastBoard.cpp:2650 (branching cap)
 
 struct Bits {
@@ -146,91 +148,234 @@ __device__ Bits bits_fill(Bits s, const Bits &mask, int W) {
     }
 }
 
-// ---- the readers' board -------------------------------------------------
+// ---- the board in LDS ----------------------------------------------------
+// One board per workgroup (one wavefront): the record's squares and string
+// representatives, every string's stones by representative, liberty counts,
+// after-liberties, ladder flags and the ladder task list.
 
-struct RBoard {
-    Bits c[2];  // black, white stones
-};
+__shared__ uint8_t s_sq[kEncGrid];
+__shared__ uint16_t s_head[kEncGrid];
+__shared__ uint16_t s_libs[kEncGrid];             // by representative
+__shared__ uint8_t s_aft[2][kEncGrid];            // after-liberties, capped 255
+__shared__ uint32_t s_lad[kEncGrid];              // bit 0 losing, bit 1 winning
+__shared__ uint64_t s_strings[kEncGrid][kWords];  // stones by representative
+__shared__ uint32_t s_tasks[5 * 361];             // <= 4 winning + 1 losing per point
+__shared__ int s_ntasks;
 
-struct Geo {
-    Bits on;    // on-board points
-    int W;
-    int d[4];   // FastBoard::m_dirs order: -W, +1, +W, -1
-};
-
-__device__ __forceinline__ Bits empties(const RBoard &b, const Geo &g) {
-    Bits e;
+__device__ __forceinline__ Bits base_string(int h) {
+    Bits s;
 #pragma unroll
-    for (int i = 0; i < kWords; i++)
-        e.w[i] = g.on.w[i] & ~(b.c[0].w[i] | b.c[1].w[i]);
-    return e;
+    for (int i = 0; i < kWords; i++) s.w[i] = s_strings[h][i];
+    return s;
 }
 
-__device__ __forceinline__ int colour_at(const RBoard &b, const Geo &g, int v) {
-    if (bits_test(b.c[0], v)) return kBlack;
-    if (bits_test(b.c[1], v)) return kWhite;
-    return bits_test(g.on, v) ? kEmpty : kOff;
+// ---- the readers: one ladder task per group of 8 lanes ------------------
+// A reading works on its own copy of the board. Its sets are spread over
+// the 8 lanes of a group: lane j < 7 holds word j (points 64j..64j+63),
+// lane 7 holds none. A set operation is one instruction per lane; the
+// neighbours of a set need the words of lanes j-1 and j+1 (DPP row
+// shifts); counts, first members and point tests are reduced within the
+// group (DPP quad permutes and the half-row mirror). Every value a branch
+// depends on is such a reduction, so the 8 lanes of a group always take
+// the same path; different groups may diverge.
+
+constexpr int kDppRowShl1 = 0x101;
+constexpr int kDppRowShr1 = 0x111;
+constexpr int kDppQuadXor1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;
+
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, 0xF, 0xF,
+                                                 true);
 }
 
-__device__ __forceinline__ Bits string_at(const RBoard &b, const Geo &g,
-                                          int col, int v) {
-    return bits_fill(bits_point(v), b.c[col], g.W);
+template <int kCtrl>
+__device__ __forceinline__ uint64_t dpp64(uint64_t x) {
+    const uint32_t lo = dpp<kCtrl>((uint32_t)x);
+    const uint32_t hi = dpp<kCtrl>((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ int libs_of(const Bits &s, const Bits &E,
-                                       const Geo &g) {
-    return bits_count(bits_and(bits_nbrs(s, g.W), E));
+// Sum and minimum over the 8 lanes of the group.
+__device__ __forceinline__ int gsum(int x) {
+    x += (int)dpp<kDppQuadXor1>((uint32_t)x);
+    x += (int)dpp<kDppQuadXor2>((uint32_t)x);
+    x += (int)dpp<kDppHalfMirror>((uint32_t)x);
+    return x;
 }
 
-__device__ __forceinline__ int empty_nbrs(const Bits &E, const Geo &g, int v) {
+__device__ __forceinline__ int gmin(int x) {
+    int y = (int)dpp<kDppQuadXor1>((uint32_t)x);
+    x = y < x ? y : x;
+    y = (int)dpp<kDppQuadXor2>((uint32_t)x);
+    x = y < x ? y : x;
+    y = (int)dpp<kDppHalfMirror>((uint32_t)x);
+    return y < x ? y : x;
+}
+
+struct Grp {
+    int lane;      // lane within the wavefront
+    int gl;        // this lane's word (lane within the group)
+    int W;         // grid width
+    int d[4];      // FastBoard::m_dirs order: -W, +1, +W, -1
+    uint64_t on;   // this lane's word of the on-board points
+};
+
+struct WBoard {
+    uint64_t c[2];  // this lane's word of the black and white stones
+    // A connected set of stones of colour hc known to the reading (the
+    // escaping string) and its neighbours: fills through a stone of it, or
+    // next to it, start from the whole set.
+    uint64_t h, hn;
+    int hc;
+};
+
+__device__ __forceinline__ uint64_t w_point(const Grp &g, int v) {
+    return g.gl == (v >> 6) ? 1ull << (v & 63) : 0ull;
+}
+
+// The group's 8 bits of a ballot (other groups' lanes, active or not,
+// are not read).
+__device__ __forceinline__ uint32_t group_ballot(bool p, int lane) {
+    return (uint32_t)(__builtin_amdgcn_ballot_w64(p) >> (lane & ~7)) & 0xFFu;
+}
+
+__device__ __forceinline__ bool w_any(uint64_t s, const Grp &g) {
+    return group_ballot(s != 0, g.lane) != 0;
+}
+
+__device__ __forceinline__ int w_count(uint64_t s) {
+    return gsum(__popcll(s));
+}
+
+__device__ __forceinline__ bool w_test(uint64_t s, const Grp &g, int v) {
+    return group_ballot(g.gl == (v >> 6) && ((s >> (v & 63)) & 1ull), g.lane) != 0;
+}
+
+// Lowest member, -1 for the empty set.
+__device__ __forceinline__ int w_first(uint64_t s, const Grp &g) {
+    const int big = 0x7fffffff;
+    const int m = gmin(s ? 64 * g.gl + __builtin_ctzll(s) : big);
+    return m == big ? -1 : m;
+}
+
+__device__ __forceinline__ uint64_t w_nbrs(uint64_t s, const Grp &g) {
+    uint64_t lo = dpp64<kDppRowShr1>(s);
+    const uint64_t hi = dpp64<kDppRowShl1>(s);
+    if (g.gl == 0) lo = 0;
+    const int W = g.W;
+    const uint64_t n = (s << 1) | (lo >> 63) | (s >> 1) | (hi << 63) |
+                       (s << W) | (lo >> (64 - W)) | (s >> W) |
+                       (hi << (64 - W));
+    return g.gl == 7 ? 0ull : n;  // lane 7 holds no points
+}
+
+__device__ __forceinline__ uint64_t w_empties(const WBoard &b, const Grp &g) {
+    return g.on & ~(b.c[0] | b.c[1]);
+}
+
+__device__ uint64_t w_fill(uint64_t s, uint64_t mask, const Grp &g) {
+    for (;;) {
+        const uint64_t t = s | (w_nbrs(s, g) & mask);
+        if (!w_any(t ^ s, g)) return t;
+        s = t;
+    }
+}
+
+// Colours of v's four neighbours, 2 bits each in m_dirs order (one
+// reduction: each neighbour lies in one lane's word).
+__device__ int nbr_colours(const WBoard &b, const Grp &g, int v) {
+    int x = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int p = v + g.d[k];
+        if (g.gl == (p >> 6)) {
+            const int sh = p & 63;
+            const int c = ((b.c[0] >> sh) & 1ull)   ? kBlack
+                          : ((b.c[1] >> sh) & 1ull) ? kWhite
+                          : ((g.on >> sh) & 1ull)   ? kEmpty
+                                                    : kOff;
+            x |= c << (2 * k);
+        }
+    }
+    return gsum(x);
+}
+
+__device__ __forceinline__ int field(int nc, int k) { return (nc >> (2 * k)) & 3; }
+
+__device__ __forceinline__ int count_empty(int nc) {
     int n = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) n += bits_test(E, v + g.d[k]);
+    for (int k = 0; k < 4; k++) n += field(nc, k) == kEmpty;
     return n;
+}
+
+// The string through the stone of colour col at v. The fill starts from
+// the board's own string at v when that string is still on the reader's
+// board whole: the reader only adds stones and captures whole strings, so
+// then it is a connected part of the answer, and the fill ends after the
+// stones added since.
+// The same holds for the reading's own escaping string (b.h), which only
+// grows while it is read.
+__device__ uint64_t string_at(const WBoard &b, const Grp &g, int col, int v) {
+    uint64_t seed = w_point(g, v);
+    if (s_sq[v] == col) {
+        const uint64_t base = g.gl < kWords ? s_strings[s_head[v]][g.gl] : 0ull;
+        if (!w_any(base & ~b.c[col], g)) seed |= base;
+    }
+    if (col == b.hc && w_test(b.h | b.hn, g, v) && !w_any(b.h & ~b.c[col], g))
+        seed |= b.h;
+    return w_fill(seed, b.c[col], g);
+}
+
+__device__ __forceinline__ void set_hint(WBoard &b, const Grp &g, int col,
+                                         uint64_t s) {
+    b.h = s;
+    b.hn = w_nbrs(s, g);
+    b.hc = col;
+}
+
+__device__ __forceinline__ int libs_of(uint64_t s, uint64_t E, const Grp &g) {
+    return w_count(w_nbrs(s, g) & E);
 }
 
 // FastBoard::update_board_fast (the readers' move, FastBoard.cpp:812-870):
 // captures opponent strings left without liberties; the played string is
 // removed when it has none, unless the stone went into an opponent eye.
-__device__ void play_fast(RBoard &b, const Geo &g, int col, int v) {
+__device__ void play_fast(WBoard &b, const Grp &g, int col, int v) {
+    const int nc = nbr_colours(b, g, v);
     bool eyeplay = true;
 #pragma unroll
+    for (int k = 0; k < 4; k++)
+        eyeplay &= field(nc, k) == (col ^ 1) || field(nc, k) == kOff;
+    b.c[col] |= w_point(g, v);
+    uint64_t E = w_empties(b, g);
     for (int k = 0; k < 4; k++) {
-        const int s = colour_at(b, g, v + g.d[k]);
-        eyeplay &= s == (col ^ 1) || s == kOff;
-    }
-    bits_set(b.c[col], v);
-    Bits E = empties(b, g);
-    for (int k = 0; k < 4; k++) {
+        if (field(nc, k) != (col ^ 1)) continue;
         const int a = v + g.d[k];
-        if (!bits_test(b.c[col ^ 1], a)) continue;
-        const Bits t = string_at(b, g, col ^ 1, a);
-        if (!bits_any(bits_and(bits_nbrs(t, g.W), E))) {
-            b.c[col ^ 1] = bits_andnot(b.c[col ^ 1], t);
-            E = bits_or(E, t);
+        if (!w_test(b.c[col ^ 1], g, a)) continue;  // taken with an earlier one
+        const uint64_t t = string_at(b, g, col ^ 1, a);
+        if (!w_any(w_nbrs(t, g) & E, g)) {
+            b.c[col ^ 1] &= ~t;
+            E |= t;
         }
     }
     if (!eyeplay) {
-        const Bits m = string_at(b, g, col, v);
-        if (!bits_any(bits_and(bits_nbrs(m, g.W), E)))
-            b.c[col] = bits_andnot(b.c[col], m);
+        const uint64_t m = string_at(b, g, col, v);
+        if (!w_any(w_nbrs(m, g) & E, g)) b.c[col] &= ~m;
     }
 }
 
-// Liberties of the string through the stone at v (v must hold a stone).
-__device__ __forceinline__ int string_libs(const RBoard &b, const Geo &g,
-                                           int col, int v) {
-    return libs_of(string_at(b, g, col, v), empties(b, g), g);
-}
-
 // FastBoard::minimum_elib_count (FastBoard.cpp:2429-2443).
-__device__ int min_enemy_libs(const RBoard &b, const Geo &g, int col, int v) {
+__device__ int min_enemy_libs(const WBoard &b, const Grp &g, int col, int v) {
+    const int nc = nbr_colours(b, g, v);
+    const uint64_t E = w_empties(b, g);
     int m = 100;
-    const Bits E = empties(b, g);
     for (int k = 0; k < 4; k++) {
-        const int a = v + g.d[k];
-        if (!bits_test(b.c[col ^ 1], a)) continue;
-        const int l = libs_of(string_at(b, g, col ^ 1, a), E, g);
+        if (field(nc, k) != (col ^ 1)) continue;
+        const int l = libs_of(string_at(b, g, col ^ 1, v + g.d[k]), E, g);
         m = l < m ? l : m;
     }
     return m;
@@ -239,66 +384,63 @@ __device__ int min_enemy_libs(const RBoard &b, const Geo &g, int col, int v) {
 // FastBoard::is_suicide (FastBoard.cpp:191-240) for an empty v: no empty
 // neighbour, every own neighbour string has v as its only liberty, and no
 // opponent neighbour string is in atari.
-__device__ bool is_suicide(const RBoard &b, const Geo &g, int v, int col) {
-    const Bits E = empties(b, g);
-    if (empty_nbrs(E, g, v)) return false;
+__device__ bool is_suicide(const WBoard &b, const Grp &g, int v, int col) {
+    const int nc = nbr_colours(b, g, v);
+    if (count_empty(nc)) return false;
+    const uint64_t E = w_empties(b, g);
     for (int k = 0; k < 4; k++) {
-        const int a = v + g.d[k];
-        if (bits_test(b.c[col], a)) {
-            if (libs_of(string_at(b, g, col, a), E, g) != 1) return false;
-        } else if (bits_test(b.c[col ^ 1], a)) {
-            if (libs_of(string_at(b, g, col ^ 1, a), E, g) <= 1) return false;
-        }
+        const int s = field(nc, k);
+        if (s != kBlack && s != kWhite) continue;
+        const int l = libs_of(string_at(b, g, s, v + g.d[k]), E, g);
+        if (s == col ? l != 1 : l <= 1) return false;
     }
     return true;
 }
 
 // FastBoard::self_atari (FastBoard.cpp:1401-1471): with v played the
 // string's liberties, v included, number at most two.
-__device__ bool self_atari(const RBoard &b, const Geo &g, int col, int v) {
-    const Bits E = empties(b, g);
-    if (empty_nbrs(E, g, v) >= 2) return false;
+__device__ bool self_atari(const WBoard &b, const Grp &g, int col, int v) {
+    const int nc = nbr_colours(b, g, v);
+    if (count_empty(nc) >= 2) return false;
+    const uint64_t E = w_empties(b, g);
     int own_or_edge = 0;
-    Bits u = bits_point(v);
+    uint64_t u = w_point(g, v);
     for (int k = 0; k < 4; k++) {
-        const int a = v + g.d[k];
-        const int s = colour_at(b, g, a);
+        const int s = field(nc, k), a = v + g.d[k];
         if (s == kOff) {
             own_or_edge++;
         } else if (s == kEmpty) {
-            bits_set(u, a);
+            u |= w_point(g, a);
         } else {
-            const Bits t = string_at(b, g, s, a);
-            const Bits lt = bits_and(bits_nbrs(t, g.W), E);
-            const int l = bits_count(lt);
+            const uint64_t lt = w_nbrs(string_at(b, g, s, a), g) & E;
+            const int l = w_count(lt);
             if (s == col) {
                 own_or_edge++;
                 if (l >= 3) return false;  // connects out
-                if (l > 1) u = bits_or(u, lt);
+                if (l > 1) u |= lt;
             } else if (l <= 1) {
                 return false;  // captures
             }
         }
     }
     if (own_or_edge == 0) return true;
-    return bits_count(u) <= 2;
+    return w_count(u) <= 2;
 }
 
 // FastBoard::after_liberties_color (FastBoard.cpp:2511-2528): liberties of
 // the string through v after col plays it, 0 when that is suicide.
-__device__ int after_libs(const RBoard &b, const Geo &g, int col, int v) {
+__device__ int after_libs(const WBoard &b, const Grp &g, int col, int v) {
     if (is_suicide(b, g, v, col)) return 0;
-    RBoard t = b;
+    WBoard t = b;
     play_fast(t, g, col, v);
-    return string_libs(t, g, col, v);
+    return libs_of(string_at(t, g, col, v), w_empties(t, g), g);
 }
 
 // The only liberty of the string through the stone at v, 0 when it has
 // another number of liberties (FastBoard::in_atari).
-__device__ int atari_liberty(const RBoard &b, const Geo &g, int col, int v) {
-    const Bits l = bits_and(bits_nbrs(string_at(b, g, col, v), g.W),
-                            empties(b, g));
-    return bits_count(l) == 1 ? bits_first(l) : 0;
+__device__ int atari_liberty(const WBoard &b, const Grp &g, int col, int v) {
+    const uint64_t l = w_nbrs(string_at(b, g, col, v), g) & w_empties(b, g);
+    return w_count(l) == 1 ? w_first(l, g) : 0;
 }
 
 // One pass of FastBoard::check_losing_ladder (FastBoard.cpp:2647-2837) for
@@ -306,7 +448,7 @@ __device__ int atari_liberty(const RBoard &b, const Geo &g, int col, int v) {
 // Returns 1 (the escape fails), 0 (it works), or 2 at a tie between the
 // attacker's two ataris: then b is the board before the attacker's move
 // and lib0/lib1 the two candidate points.
-__device__ int ladder_pass(RBoard &b, const Geo &g, int col, int v,
+__device__ int ladder_pass(WBoard &b, const Grp &g, int col, int v,
                            int branching, int &lib0, int &lib1) {
     if (branching > kMaxBranch) return 0;
     const int att = col ^ 1;
@@ -314,46 +456,47 @@ __device__ int ladder_pass(RBoard &b, const Geo &g, int col, int v,
     if (elib == 0 || elib == 1) return 0;  // an attacking string dies
     // The own strings in atari next to v: exactly one.
     {
-        const Bits E = empties(b, g);
-        Bits crit = bits_zero();
+        const int nc = nbr_colours(b, g, v);
+        const uint64_t E = w_empties(b, g);
+        uint64_t crit = 0;
         int ncrit = 0;
         for (int k = 0; k < 4; k++) {
+            if (field(nc, k) != col) continue;
             const int a = v + g.d[k];
-            if (!bits_test(b.c[col], a)) continue;
-            const Bits t = string_at(b, g, col, a);
+            const uint64_t t = string_at(b, g, col, a);
             if (libs_of(t, E, g) > 1) continue;
             if (ncrit == 0) {
                 crit = t;
                 ncrit = 1;
-            } else if (!bits_test(crit, a)) {
+            } else if (!w_test(crit, g, a)) {
                 ncrit++;
             }
         }
         if (ncrit != 1) return 0;
         // FastBoard::can_kill_neighbours (FastBoard.cpp:2587-2613).
-        Bits adj = bits_and(bits_nbrs(crit, g.W), b.c[att]);
-        while (bits_any(adj)) {
-            const Bits t = string_at(b, g, att, bits_first(adj));
+        uint64_t adj = w_nbrs(crit, g) & b.c[att];
+        while (w_any(adj, g)) {
+            const uint64_t t = string_at(b, g, att, w_first(adj, g));
             if (libs_of(t, E, g) <= 1) return 0;
-            adj = bits_andnot(adj, t);
+            adj &= ~t;
         }
     }
     int atari = v;
     play_fast(b, g, col, v);
     for (;;) {
-        if (!bits_test(b.c[col], atari)) return 1;  // the escape was suicide
-        const Bits E = empties(b, g);
-        const Bits s = string_at(b, g, col, atari);
-        const Bits ls = bits_and(bits_nbrs(s, g.W), E);
-        const int nl = bits_count(ls);
+        if (!w_test(b.c[col], g, atari)) return 1;  // the escape was suicide
+        const uint64_t E = w_empties(b, g);
+        const uint64_t s = string_at(b, g, col, atari);
+        set_hint(b, g, col, s);
+        const uint64_t ls = w_nbrs(s, g) & E;
+        const int nl = w_count(ls);
         if (nl == 1) return 1;  // still in atari
         if (nl >= 3) return 0;  // escaped
         if (min_enemy_libs(b, g, col, atari) == 1) return 0;
-        const int l0p = bits_first(ls);
-        Bits rest = ls;
-        bits_clear(rest, l0p);
-        const int l1p = bits_first(rest);
-        if (empty_nbrs(E, g, l0p) == 3 && empty_nbrs(E, g, l1p) == 3)
+        const int l0p = w_first(ls, g);
+        const int l1p = w_first(ls & ~w_point(g, l0p), g);
+        if (count_empty(nbr_colours(b, g, l0p)) == 3 &&
+            count_empty(nbr_colours(b, g, l1p)) == 3)
             return 0;  // two open liberties
         int l0 = after_libs(b, g, col, l0p);
         int l1 = after_libs(b, g, col, l1p);
@@ -375,16 +518,16 @@ __device__ int ladder_pass(RBoard &b, const Geo &g, int col, int v,
 }
 
 struct Pending {
-    RBoard b;
+    WBoard b;
     int v;
     int branching;
 };
 
 // FastBoard::check_losing_ladder with its recursion as a depth-first
 // search: at a tie the result is the OR over the attacker's two ataris,
-// each continuing one level deeper. At most one alternative per level is
-// pending, so the stack holds kMaxBranch + 1 boards.
-__device__ bool losing_ladder(RBoard b, const Geo &g, int col, int v) {
+// each continuing one level deeper. Pending depths grow strictly from the
+// bottom of the stack, so it holds at most kMaxBranch + 1 boards.
+__device__ bool losing_ladder(WBoard b, const Grp &g, int col, int v) {
     Pending stack[kMaxBranch + 1];
     int top = 0;
     int branching = 0;
@@ -414,10 +557,10 @@ __device__ bool losing_ladder(RBoard b, const Geo &g, int col, int v) {
 // The trial of FastBoard::check_winning_ladder (FastBoard.cpp:2530-2564)
 // against the opponent string through a: col ataris at v, and the escape
 // (its last liberty, with two empty neighbours) reads as a losing ladder.
-__device__ bool winning_trial(RBoard b, const Geo &g, int col, int v, int a) {
+__device__ bool winning_trial(WBoard b, const Grp &g, int col, int v, int a) {
     play_fast(b, g, col, v);
     const int escape = atari_liberty(b, g, col ^ 1, a);
-    if (!escape || empty_nbrs(empties(b, g), g, escape) != 2) return false;
+    if (!escape || count_empty(nbr_colours(b, g, escape)) != 2) return false;
     return losing_ladder(b, g, col ^ 1, escape);
 }
 
@@ -450,17 +593,9 @@ __device__ __forceinline__ PlaneMap plane_map(int net) {
 constexpr int kTaskLose = 1 << 20;
 
 __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
-                                                    int net, int symmetry,
-                                                    uint8_t *out) {
-    __shared__ uint8_t sq[kEncGrid];
-    __shared__ uint16_t head[kEncGrid];
-    __shared__ uint16_t libs[kEncGrid];        // by representative
-    __shared__ uint8_t aft[2][kEncGrid];       // after-liberties, capped 255
-    __shared__ uint32_t lad[kEncGrid];         // bit 0 losing, bit 1 winning
-    __shared__ uint64_t strings[kEncGrid][kWords];  // stones by representative
-    __shared__ uint32_t tasks[kMaxTasks];
-    __shared__ int ntasks;
-
+                                                    int symmetry,
+                                                    uint8_t *out_policy,
+                                                    uint8_t *out_value) {
     const int lane = threadIdx.x;
     const EncRecord &r = recs[blockIdx.x];
     const int side = r.side, W = side + 2, G = W * W, P = side * side;
@@ -469,69 +604,64 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
     // 1. Stage the squares and representatives; ballots give the boards.
     {
         const uint32_t *s32 = reinterpret_cast<const uint32_t *>(r.sq);
-        uint32_t *d32 = reinterpret_cast<uint32_t *>(sq);
+        uint32_t *d32 = reinterpret_cast<uint32_t *>(s_sq);
         for (int i = lane; i < kEncGrid / 4; i += 64) d32[i] = s32[i];
         const uint32_t *h32 = reinterpret_cast<const uint32_t *>(r.head);
-        uint32_t *e32 = reinterpret_cast<uint32_t *>(head);
+        uint32_t *e32 = reinterpret_cast<uint32_t *>(s_head);
         for (int i = lane; i < kEncGrid / 2; i += 64) e32[i] = h32[i];
     }
-    if (lane == 0) ntasks = 0;
+    if (lane == 0) s_ntasks = 0;
     __syncthreads();
-    Geo g;
-    RBoard board;
-    g.W = W;
-    g.d[0] = -W;
-    g.d[1] = 1;
-    g.d[2] = W;
-    g.d[3] = -1;
+    const int d[4] = {-W, 1, W, -1};  // FastBoard::m_dirs order
+    Bits stones[2], on;
 #pragma unroll
     for (int i = 0; i < kWords; i++) {
         const int v = 64 * i + lane;
-        const int c = v < G ? sq[v] : kOff;
-        board.c[0].w[i] = __ballot(c == kBlack);
-        board.c[1].w[i] = __ballot(c == kWhite);
-        g.on.w[i] = __ballot(c != kOff);
+        const int c = v < G ? s_sq[v] : kOff;
+        stones[0].w[i] = __ballot(c == kBlack);
+        stones[1].w[i] = __ballot(c == kWhite);
+        on.w[i] = __ballot(c != kOff);
     }
-    const Bits E = empties(board, g);
+    const Bits E = bits_andnot(on, bits_or(stones[0], stones[1]));
 
     // 2. Stones of every string, then its liberty count.
     for (int v = lane; v < G; v += 64) {
-        lad[v] = 0;
-        const int c = sq[v];
-        if ((c == kBlack || c == kWhite) && head[v] == v) {
+        s_lad[v] = 0;
+        const int c = s_sq[v];
+        if ((c == kBlack || c == kWhite) && s_head[v] == v) {
 #pragma unroll
-            for (int i = 0; i < kWords; i++) strings[v][i] = 0;
+            for (int i = 0; i < kWords; i++) s_strings[v][i] = 0;
         }
     }
     __syncthreads();
     for (int v = lane; v < G; v += 64) {
-        const int c = sq[v];
+        const int c = s_sq[v];
         if (c == kBlack || c == kWhite)
             atomicOr(reinterpret_cast<unsigned long long *>(
-                         &strings[head[v]][v >> 6]),
+                         &s_strings[s_head[v]][v >> 6]),
                      1ull << (v & 63));
     }
     __syncthreads();
     for (int v = lane; v < G; v += 64) {
-        const int c = sq[v];
-        if ((c == kBlack || c == kWhite) && head[v] == v) {
+        const int c = s_sq[v];
+        if ((c == kBlack || c == kWhite) && s_head[v] == v) {
             Bits s;
 #pragma unroll
-            for (int i = 0; i < kWords; i++) s.w[i] = strings[v][i];
-            libs[v] = (uint16_t)libs_of(s, E, g);
+            for (int i = 0; i < kWords; i++) s.w[i] = s_strings[v][i];
+            s_libs[v] = (uint16_t)bits_count(bits_and(bits_nbrs(s, W), E));
         }
     }
     __syncthreads();
 
     // 3. Lane per empty point: after-liberties and ladder tasks.
     for (int v = lane; v < G; v += 64) {
-        if (sq[v] != kEmpty) continue;
+        if (s_sq[v] != kEmpty) continue;
         int nh[4], ncol[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const int a = v + g.d[k];
-            ncol[k] = sq[a];
-            nh[k] = (ncol[k] == kBlack || ncol[k] == kWhite) ? head[a] : -1;
+            const int a = v + d[k];
+            ncol[k] = s_sq[a];
+            nh[k] = (ncol[k] == kBlack || ncol[k] == kWhite) ? s_head[a] : -1;
         }
         // After-liberties (BoardBits formula): the merged string M = v plus
         // the adjacent strings of the mover; its liberties are M's
@@ -548,16 +678,16 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
                 if (nh[k] < 0) continue;
                 Bits s;
 #pragma unroll
-                for (int i = 0; i < kWords; i++) s.w[i] = strings[nh[k]][i];
+                for (int i = 0; i < kWords; i++) s.w[i] = s_strings[nh[k]][i];
                 if (ncol[k] == col)
                     m = bits_or(m, s);
-                else if (libs[nh[k]] == 1)
+                else if (s_libs[nh[k]] == 1)
                     e = bits_or(e, s);
             }
             after[t] = bits_count(bits_and(bits_nbrs(m, W), e));
         }
-        aft[0][v] = (uint8_t)(after[0] > 255 ? 255 : after[0]);
-        aft[1][v] = (uint8_t)(after[1] > 255 ? 255 : after[1]);
+        s_aft[0][v] = (uint8_t)(after[0] > 255 ? 255 : after[0]);
+        s_aft[1][v] = (uint8_t)(after[1] > 255 ? 255 : after[1]);
 
         int en = 0;
         bool own_atari = false, captures = false, kill = false, connect = false;
@@ -567,7 +697,7 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
             en += ncol[k] == kEmpty;
             if (ncol[k] == kOff) own_or_edge++;
             if (nh[k] < 0) continue;
-            const int l = libs[nh[k]];
+            const int l = s_libs[nh[k]];
             if (ncol[k] == tm) {
                 own_or_edge++;
                 own_atari |= l == 1;
@@ -581,8 +711,8 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
         // neighbours and a saving size: an own neighbour string in atari
         // (v, its liberty, is no self-atari with two empty neighbours).
         if (en == 2 && own_atari) {
-            const int i = atomicAdd(&ntasks, 1);
-            tasks[i] = (uint32_t)v | kTaskLose;
+            const int i = atomicAdd(&s_ntasks, 1);
+            s_tasks[i] = (uint32_t)v | kTaskLose;
         }
         // Winning ladder: ataris on adjacent two-liberty opponent strings
         // that are not self-atari; without a capture the escape must have
@@ -595,12 +725,12 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
         } else {
             Bits u = bits_point(v);
             for (int k = 0; k < 4; k++) {
-                const int a = v + g.d[k];
+                const int a = v + d[k];
                 if (ncol[k] == kEmpty) bits_set(u, a);
-                if (nh[k] >= 0 && ncol[k] == tm && libs[nh[k]] > 1) {
+                if (nh[k] >= 0 && ncol[k] == tm && s_libs[nh[k]] > 1) {
                     Bits s;
 #pragma unroll
-                    for (int i = 0; i < kWords; i++) s.w[i] = strings[nh[k]][i];
+                    for (int i = 0; i < kWords; i++) s.w[i] = s_strings[nh[k]][i];
                     u = bits_or(u, bits_and(bits_nbrs(s, W), E));
                 }
             }
@@ -608,84 +738,114 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
         }
         if (sa) continue;
         for (int k = 0; k < 4; k++) {
-            if (ncol[k] != opp || libs[nh[k]] != 2) continue;
+            if (ncol[k] != opp || s_libs[nh[k]] != 2) continue;
             bool dup = false;
             for (int j = 0; j < k; j++) dup |= nh[j] == nh[k];
             if (dup) continue;
             if (!captures) {
                 Bits s;
 #pragma unroll
-                for (int i = 0; i < kWords; i++) s.w[i] = strings[nh[k]][i];
+                for (int i = 0; i < kWords; i++) s.w[i] = s_strings[nh[k]][i];
                 Bits l = bits_and(bits_nbrs(s, W), E);
                 bits_clear(l, v);
                 const int e = bits_first(l);
                 int een = 0;
                 for (int j = 0; j < 4; j++)
-                    een += sq[e + g.d[j]] == kEmpty && e + g.d[j] != v;
+                    een += s_sq[e + d[j]] == kEmpty && e + d[j] != v;
                 if (een != 2) continue;
             }
-            const int i = atomicAdd(&ntasks, 1);
-            tasks[i] = (uint32_t)v | ((uint32_t)nh[k] << 9);
+            const int i = atomicAdd(&s_ntasks, 1);
+            s_tasks[i] = (uint32_t)v | ((uint32_t)nh[k] << 9);
         }
     }
     __syncthreads();
 
-    // 4. Lane per ladder task.
-    const int nt = ntasks;
-    for (int t = lane; t < nt; t += 64) {
-        const uint32_t task = tasks[t];
-        const int v = task & 511;
-        if (task & kTaskLose) {
-            if (losing_ladder(board, g, tm, v)) atomicOr(&lad[v], 1u);
-        } else {
-            const int a = (task >> 9) & 511;
-            if (winning_trial(board, g, tm, v, a)) atomicOr(&lad[v], 2u);
+    // 4. Group of 8 lanes per ladder task.
+    {
+        Grp g;
+        g.lane = lane;
+        g.gl = lane & 7;
+        g.W = W;
+#pragma unroll
+        for (int k = 0; k < 4; k++) g.d[k] = d[k];
+        WBoard wb;
+        g.on = wb.c[0] = wb.c[1] = wb.h = wb.hn = 0;
+        wb.hc = kEmpty;
+#pragma unroll
+        for (int i = 0; i < kWords; i++) {
+            if (g.gl == i) {
+                g.on = on.w[i];
+                wb.c[0] = stones[0].w[i];
+                wb.c[1] = stones[1].w[i];
+            }
+        }
+        const int nt = s_ntasks;
+        for (int t = lane >> 3; t < nt; t += 8) {
+            const uint32_t task = s_tasks[t];
+            const int v = task & 511;
+            bool hit;
+            uint32_t bit;
+            if (task & kTaskLose) {
+                hit = losing_ladder(wb, g, tm, v);
+                bit = 1u;
+            } else {
+                hit = winning_trial(wb, g, tm, v, (task >> 9) & 511);
+                bit = 2u;
+            }
+            if (hit && g.gl == 0) atomicOr(&s_lad[v], bit);
         }
     }
     __syncthreads();
 
-    // 5. Lane per output index: the 32 plane bytes of the rotated point.
-    const PlaneMap L = plane_map(net);
-    uint8_t *o = out + r.out_offset;
+    // 5. Lane per output index: the plane bytes of the rotated point, for
+    // each net asked for.
     for (int i = lane; i < P; i += 64) {
         const int src = rotate_idx(i, symmetry, side);
         const int x = src % side, y = src / side;
         const int v = (y + 1) * W + (x + 1);
-        const int c = sq[v];
-        uint32_t m = 0;
-        if (x == 2 || x == side - 3 || y == 2 || y == side - 3)
-            m |= 1u << L.line3;
-        if (c == kEmpty) {
-            m |= 1u;
-            const int a0 = aft[0][v], a1 = aft[1][v];
-            if (a0 >= 1) m |= 1u << (L.after_own + (a0 >= 6 ? 5 : a0 - 1));
-            if (a1 >= 1) m |= 1u << (L.after_opp + (a1 >= 6 ? 5 : a1 - 1));
-            if (lad[v] & 1u) m |= 1u << L.ladder;
-            if (lad[v] & 2u) m |= 1u << L.ladder_win;
-        } else {
-            if (c == kWhite && r.white_has_komi) m |= 1u << L.has_komi;
-            m |= c == tm ? 2u : 4u;
-            int l = libs[head[v]];
-            if (l > L.lib_top) l = L.lib_top;
-            if (l >= 1) m |= 1u << ((c == tm ? L.own_libs : L.opp_libs) + l - 1);
-        }
-        if (r.komove > 0 && v == r.komove) m |= 1u << L.komove;
-        if (L.last >= 0 && r.last_move > 0) {
-            if (v == r.last_move) m |= 1u << L.last;
-            if (r.prev_last > 0 && v == r.prev_last) m |= 1u << L.prev;
-        }
+        const int c = s_sq[v];
+        const bool line3 = x == 2 || x == side - 3 || y == 2 || y == side - 3;
+        const int libs = c == kEmpty ? 0 : s_libs[s_head[v]];
+        for (int net = 0; net < 2; net++) {
+            uint8_t *out = net == 0 ? out_policy : out_value;
+            if (!out) continue;
+            const PlaneMap L = plane_map(net);
+            uint32_t m = line3 ? 1u << L.line3 : 0u;
+            if (c == kEmpty) {
+                m |= 1u;
+                const int a0 = s_aft[0][v], a1 = s_aft[1][v];
+                if (a0 >= 1) m |= 1u << (L.after_own + (a0 >= 6 ? 5 : a0 - 1));
+                if (a1 >= 1) m |= 1u << (L.after_opp + (a1 >= 6 ? 5 : a1 - 1));
+                if (s_lad[v] & 1u) m |= 1u << L.ladder;
+                if (s_lad[v] & 2u) m |= 1u << L.ladder_win;
+            } else {
+                if (c == kWhite && r.white_has_komi) m |= 1u << L.has_komi;
+                m |= c == tm ? 2u : 4u;
+                const int l = libs > L.lib_top ? L.lib_top : libs;
+                if (l >= 1)
+                    m |= 1u << ((c == tm ? L.own_libs : L.opp_libs) + l - 1);
+            }
+            if (r.komove > 0 && v == r.komove) m |= 1u << L.komove;
+            if (L.last >= 0 && r.last_move > 0) {
+                if (v == r.last_move) m |= 1u << L.last;
+                if (r.prev_last > 0 && v == r.prev_last) m |= 1u << L.prev;
+            }
+            uint8_t *o = out + r.out_offset;
 #pragma unroll
-        for (int p = 0; p < 32; p++) o[(size_t)p * P + i] = (uint8_t)((m >> p) & 1u);
+            for (int p = 0; p < 32; p++)
+                o[(size_t)p * P + i] = (uint8_t)((m >> p) & 1u);
+        }
     }
 }
 
 }  // namespace
 
-hipError_t launch_encode(const EncRecord *d_rec, int n, int net, int symmetry,
-                         uint8_t *d_out, hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(encode_kernel, dim3(n), dim3(64), 0, stream, d_rec, net,
-                       symmetry, d_out);
+hipError_t launch_encode(const EncRecord *d_rec, int n, int symmetry,
+                         uint8_t *d_policy, uint8_t *d_value,
+                         hipStream_t stream) {
+    if (n <= 0 || (!d_policy && !d_value)) return hipSuccess;
+    hipLaunchKernelGGL(encode_kernel, dim3(n), dim3(64), 0, stream, d_rec,
+                       symmetry, d_policy, d_value);
     return hipGetLastError();
 }
 

@@ -79,6 +79,10 @@ public:
     // Representative stone of the string through v, and the 4 neighbour
     // offsets of the padded grid.
     int head(int v) const { return head_[v]; }
+    // The padded grid's squares (Color values) and string representatives
+    // (meaningful at stones only), width() * width() entries each.
+    const uint8_t *squares() const { return sq_.data(); }
+    const uint16_t *heads() const { return head_.data(); }
     int dir(int k) const { return dir_[k]; }
 
 private:

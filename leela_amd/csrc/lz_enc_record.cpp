@@ -9,12 +9,8 @@ namespace lz {
 void pack_record(const go::GameState &st, int64_t offset, EncRecord *r) {
     const go::Board &b = st.board;
     const int W = b.width(), G = W * W;
-    for (int v = 0; v < G; v++) {
-        const int c = b.at(v);
-        r->sq[v] = (uint8_t)c;
-        r->head[v] = (c == go::BLACK || c == go::WHITE) ? (uint16_t)b.head(v)
-                                                        : (uint16_t)0;
-    }
+    std::memcpy(r->sq, b.squares(), G);
+    std::memcpy(r->head, b.heads(), sizeof(uint16_t) * G);
     std::memset(r->sq + G, go::OFFBOARD, kEncGrid - G);
     std::memset(r->head + G, 0, sizeof(uint16_t) * (kEncGrid - G));
     r->side = b.side();

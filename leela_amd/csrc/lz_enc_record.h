@@ -13,7 +13,7 @@ constexpr int kEncGrid = 448;  // >= 21 * 21, a multiple of 64
 
 struct alignas(64) EncRecord {
     uint8_t sq[kEncGrid];      // 0 black, 1 white, 2 empty, 3 off-board
-    uint16_t head[kEncGrid];   // string representative of each stone
+    uint16_t head[kEncGrid];   // string representative (read at stones only)
     int32_t side;              // 5..19
     int32_t to_move;           // 0 black, 1 white
     int32_t komove;            // simple-ko vertex, <= 0 when none

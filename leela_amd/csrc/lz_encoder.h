@@ -11,9 +11,11 @@
 
 namespace lz {
 
-// Planes of n boards (uint8 [32][S*S] each at out + out_offset), rotated
-// by `symmetry` (0..7, Network.cpp:765-773). net: 0 policy, 1 value.
-hipError_t launch_encode(const EncRecord *d_rec, int n, int net, int symmetry,
-                         uint8_t *d_out, hipStream_t stream);
+// Planes of n boards (uint8 [32][S*S] each at out + out_offset) for the
+// policy net's layout into d_policy and the value net's into d_value
+// (either may be NULL), rotated by `symmetry` (0..7, Network.cpp:765-773).
+hipError_t launch_encode(const EncRecord *d_rec, int n, int symmetry,
+                         uint8_t *d_policy, uint8_t *d_value,
+                         hipStream_t stream);
 
 }  // namespace lz

@@ -91,18 +91,19 @@ const char *lz_encoder_last_error(const lz_encoder *e) {
     return e ? e->err.c_str() : "null encoder";
 }
 
-int lz_encode_device(lz_encoder *e, const lz_position *const *pos, int n,
-                     int net, int symmetry, uint8_t *d_planes, void *stream) {
-    if (!e) return LZ_ERR_ARG;
-    std::lock_guard<std::mutex> g(e->mu);
+}  // extern "C"
+
+namespace {
+
+int encode(lz_encoder *e, const lz_position *const *pos, int n, int symmetry,
+           uint8_t *d_policy, uint8_t *d_value, void *stream) {
     if (n < 0 || n > e->max_batch)
         return e->fail(LZ_ERR_ARG, "batch %d outside 0..%d", n, e->max_batch);
-    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
-        return e->fail(LZ_ERR_ARG, "net %d", net);
     if (symmetry < 0 || symmetry > 7)
         return e->fail(LZ_ERR_ARG, "symmetry %d", symmetry);
     if (n == 0) return LZ_OK;
-    if (!pos || !d_planes) return e->fail(LZ_ERR_ARG, "null buffer");
+    if (!pos || (!d_policy && !d_value))
+        return e->fail(LZ_ERR_ARG, "null buffer");
     for (int i = 0; i < n; i++)
         if (!pos[i]) return e->fail(LZ_ERR_ARG, "position %d is NULL", i);
     if (hipSetDevice(e->device) != hipSuccess)
@@ -127,7 +128,7 @@ int lz_encode_device(lz_encoder *e, const lz_position *const *pos, int n,
                                     hipMemcpyHostToDevice, s);
     if (err != hipSuccess)
         return e->fail(LZ_ERR_HIP, "record copy: %s", hipGetErrorString(err));
-    err = lz::launch_encode(sl.dev, n, net, symmetry, d_planes, s);
+    err = lz::launch_encode(sl.dev, n, symmetry, d_policy, d_value, s);
     if (err != hipSuccess)
         return e->fail(LZ_ERR_HIP, "encoder launch: %s", hipGetErrorString(err));
     err = hipEventRecord(sl.done, s);
@@ -135,6 +136,31 @@ int lz_encode_device(lz_encoder *e, const lz_position *const *pos, int n,
         return e->fail(LZ_ERR_HIP, "encoder event: %s", hipGetErrorString(err));
     sl.used = true;
     return LZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lz_encode_device(lz_encoder *e, const lz_position *const *pos, int n,
+                     int net, int symmetry, uint8_t *d_planes, void *stream) {
+    if (!e) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(e->mu);
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return e->fail(LZ_ERR_ARG, "net %d", net);
+    if (n > 0 && !d_planes) return e->fail(LZ_ERR_ARG, "null buffer");
+    return encode(e, pos, n, symmetry, net == LZ_NET_POLICY ? d_planes : nullptr,
+                  net == LZ_NET_VALUE ? d_planes : nullptr, stream);
+}
+
+int lz_encode_device_both(lz_encoder *e, const lz_position *const *pos, int n,
+                          int symmetry, uint8_t *d_policy, uint8_t *d_value,
+                          void *stream) {
+    if (!e) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(e->mu);
+    if (n > 0 && (!d_policy || !d_value))
+        return e->fail(LZ_ERR_ARG, "null buffer");
+    return encode(e, pos, n, symmetry, d_policy, d_value, stream);
 }
 
 }  // extern "C"

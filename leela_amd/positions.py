@@ -56,6 +56,8 @@ def _lib():
         L.lz_encoder_last_error.restype = C.c_char_p
         L.lz_encode_device.argtypes = [vp, C.POINTER(vp), C.c_int, C.c_int,
                                        C.c_int, vp, vp]
+        L.lz_encode_device_both.argtypes = [vp, C.POINTER(vp), C.c_int,
+                                            C.c_int, vp, vp, vp]
         L._positions_bound = True
     return L
 
@@ -161,6 +163,26 @@ def encode_batch(positions, net: int = NET_POLICY, symmetry: int = 0,
     return out
 
 
+class PositionBatch:
+    """Positions with their C pointer array built once, for callers that
+    encode the same batch repeatedly (the array is what the ABI takes)."""
+
+    def __init__(self, positions):
+        self.positions = list(positions)
+        self.array = (C.c_void_p * max(len(self.positions), 1))(
+            *[p._p for p in self.positions])
+
+    def __len__(self):
+        return len(self.positions)
+
+
+def _as_array(positions):
+    if isinstance(positions, PositionBatch):
+        return positions.array, len(positions)
+    n = len(positions)
+    return (C.c_void_p * max(n, 1))(*[p._p for p in positions]), n
+
+
 class DeviceEncoder:
     """The feature encoder on the GPU (lz_encode_device): planes of a batch
     of Position objects written into device memory, one wavefront per
@@ -189,13 +211,25 @@ class DeviceEncoder:
 
     def encode_into(self, positions, d_planes: int, net: int = NET_POLICY,
                     symmetry: int = 0, stream: int = 0):
-        """Enqueue the planes of `positions` into device address d_planes
-        on `stream` (0: the HIP default stream)."""
-        n = len(positions)
-        arr = (C.c_void_p * max(n, 1))(*[p._p for p in positions])
+        """Enqueue the planes of `positions` (a list of Position or a
+        PositionBatch) into device address d_planes on `stream` (0: the
+        HIP default stream)."""
+        arr, n = _as_array(positions)
         rc = self._L.lz_encode_device(self._h, arr, n, net, symmetry,
                                       C.c_void_p(d_planes),
                                       C.c_void_p(stream))
+        if rc != LZ_OK:
+            raise LeelaError(rc, self._L.lz_encoder_last_error(self._h)
+                             .decode())
+
+    def encode_both_into(self, positions, d_policy: int, d_value: int,
+                         symmetry: int = 0, stream: int = 0):
+        """Both nets' planes from one pass (lz_encode_device_both)."""
+        arr, n = _as_array(positions)
+        rc = self._L.lz_encode_device_both(self._h, arr, n, symmetry,
+                                           C.c_void_p(d_policy),
+                                           C.c_void_p(d_value),
+                                           C.c_void_p(stream))
         if rc != LZ_OK:
             raise LeelaError(rc, self._L.lz_encoder_last_error(self._h)
                              .decode())

@@ -58,6 +58,8 @@ def emul():
     u8p = C.POINTER(C.c_uint8)
     L.emul_encode_moves.argtypes = [C.c_int, C.c_int, i32p, C.c_int, C.c_int,
                                     u8p]
+    L.emul_encode_moves_both.argtypes = [C.c_int, C.c_int, i32p, C.c_int,
+                                         u8p, u8p]
     L.emul_encode_generated.argtypes = [C.c_int, i32p,
                                         C.POINTER(C.c_uint32), i32p, C.c_int,
                                         C.c_int, u8p, u8p]
@@ -102,6 +104,20 @@ def test_kernel_symmetries(emul, gold):
             got = encode_moves(emul, side, gold[f"moves_{i}"], NET_POLICY, s)
             idx = [rot_closed_form(k, s, side) for k in range(side * side)]
             assert np.array_equal(got, base[:, idx]), (i, s)
+
+
+def test_kernel_both_layouts_in_one_pass(emul, gold):
+    """The one-pass form writes each net's planes exactly as the single-net
+    form does."""
+    for i in (3, 90, 130, 160, 200):
+        side = int(gold["cases"][i][0])
+        mv = np.ascontiguousarray(gold[f"moves_{i}"], np.int32)
+        pol = np.zeros((32, side * side), np.uint8)
+        val = np.zeros((32, side * side), np.uint8)
+        emul.emul_encode_moves_both(side, len(mv), _p(mv, C.c_int32), 2,
+                                    _p(pol, C.c_uint8), _p(val, C.c_uint8))
+        assert np.array_equal(pol, encode_moves(emul, side, mv, NET_POLICY, 2))
+        assert np.array_equal(val, encode_moves(emul, side, mv, NET_VALUE, 2))
 
 
 @pytest.mark.parametrize("net,sym", [(NET_POLICY, 0), (NET_POLICY, 5),

@@ -75,6 +75,21 @@ def test_full_batch_matches_host(enc):
         assert np.array_equal(got, encode_batch(pos, net)), net
 
 
+def test_both_nets_in_one_pass(enc):
+    import torch
+    pos = [Position.random(300 + i) for i in range(256)]
+    pol = torch.empty((256, 32, 361), dtype=torch.uint8, device="cuda")
+    val = torch.empty_like(pol)
+    for sym in (0, 6):
+        enc.encode_both_into(pos, pol.data_ptr(), val.data_ptr(), sym,
+                             torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(pol.cpu().numpy(),
+                              encode_batch(pos, NET_POLICY, sym))
+        assert np.array_equal(val.cpu().numpy(),
+                              encode_batch(pos, NET_VALUE, sym))
+
+
 def test_back_to_back_calls_reuse_slots(enc):
     """Three calls in flight on one stream (the record slots alternate);
     each output equals its own host encoding."""

@@ -15,7 +15,7 @@ using lz::go::GameState;
 namespace {
 
 void run(const std::vector<GameState> &boards, int net, int sym,
-         uint8_t *out) {
+         uint8_t *out, uint8_t *out2 = nullptr) {
     std::vector<lz::EncRecord> rec(boards.size());
     int64_t off = 0;
     for (size_t i = 0; i < boards.size(); i++) {
@@ -23,7 +23,12 @@ void run(const std::vector<GameState> &boards, int net, int sym,
         const int S = boards[i].board.side();
         off += (int64_t)lz::kPlanes * S * S;
     }
-    lz::launch_encode(rec.data(), (int)rec.size(), net, sym, out, nullptr);
+    if (out2)  // both layouts from one launch
+        lz::launch_encode(rec.data(), (int)rec.size(), sym, out, out2, nullptr);
+    else
+        lz::launch_encode(rec.data(), (int)rec.size(), sym,
+                          net == 0 ? out : nullptr, net == 1 ? out : nullptr,
+                          nullptr);
 }
 
 }  // namespace
@@ -60,6 +65,16 @@ int emul_encode_generated(int n, const int32_t *sides, const uint32_t *seeds,
         off += (int64_t)lz::kPlanes * S * S;
     }
     run(b, net, sym, out);
+    return 0;
+}
+
+// Both layouts of the board after `moves` from one launch.
+int emul_encode_moves_both(int side, int n, const int32_t *moves, int sym,
+                           uint8_t *policy, uint8_t *value) {
+    std::vector<GameState> b(1);
+    b[0].init(side, 7.5f);
+    for (int i = 0; i < n; i++) b[0].play(moves[i]);
+    run(b, 0, sym, policy, value);
     return 0;
 }
 
