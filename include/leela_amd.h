@@ -268,6 +268,16 @@ int lz_encode_device_both(lz_encoder *enc, const lz_position *const *pos,
                           int n, int symmetry, uint8_t *d_policy,
                           uint8_t *d_value, void *stream);
 
+/* lz_playout_winrate for n positions on the GPU, playouts of position i
+ * seeded seed0 + i (as lz_make_value_labels): d_winrates[i] (device
+ * memory) equals the host's value bit for bit. One group of 8 lanes per
+ * playout; the boards go through the encoder's record slots, so the same
+ * ordering rules apply. Playout::mc_owner's playouts (Playout.cpp:51-113,
+ * 174-217) for the surrogate's labels, moved onto the device. */
+int lz_playout_winrates_device(lz_encoder *enc, const lz_position *const *pos,
+                               int n, int playouts, uint32_t seed0,
+                               float *d_winrates, void *stream);
+
 /* Generator + encoder for n positions with seeds seed0..seed0+n-1:
  * planes uint8 [n][32][side*side] of `net`, rotation 0. threads <= 0 uses
  * every hardware thread. */
@@ -309,7 +319,10 @@ int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
  * Playout::run (Playout.cpp:51-113: stop at four passes or move number
  * 2*side^2, score by FastBoard::final_mc_score) with a light policy —
  * uniform over empty points that are not the ko square, an own eye or
- * suicide — in place of the reference's pattern policy. Host only. */
+ * suicide — in place of the reference's pattern policy. Playout j draws
+ * from its own stream, Random(fmix32(seed ^ j * 0x9E3779B9)) (MurmurHash3's
+ * finalizer) with two outputs discarded. Host; lz_playout_winrates_device
+ * is the same on the GPU. */
 int lz_playout_winrate(const lz_position *pos, int n, uint32_t seed,
                        float *winrate);
 

@@ -1,4 +1,4 @@
-// lz_enc_record.cpp — packing a game state into a device encoder record.
+// lz_enc_record.cpp — packing a game state into a device board record.
 #include "lz_enc_record.h"
 
 #include <cmath>
@@ -20,6 +20,9 @@ void pack_record(const go::GameState &st, int64_t offset, EncRecord *r) {
     r->prev_last = st.prev_last;
     r->white_has_komi = !(std::fabs(st.komi) <= 0.75f);
     r->out_offset = offset;
+    r->passes = st.passes;
+    r->movenum = st.movenum;
+    r->komi = st.komi;
 }
 
 }  // namespace lz

@@ -1,4 +1,5 @@
-// lz_enc_record.h — one board as the device feature encoder reads it.
+// lz_enc_record.h — one board as the device kernels read it (feature
+// encoder, playouts).
 #pragma once
 
 #include <cstdint>
@@ -21,7 +22,10 @@ struct alignas(64) EncRecord {
     int32_t prev_last;         // vertex, <= 0 when none or a pass
     int32_t white_has_komi;    // |komi| > 0.75
     int64_t out_offset;        // byte offset of this board's planes
-    uint8_t pad_[32];
+    int32_t passes;            // passes in a row (playouts stop at 4)
+    int32_t movenum;           // moves played (playouts stop at 2 side^2)
+    float komi;
+    uint8_t pad_[20];
 };
 static_assert(sizeof(EncRecord) == 1408, "record layout");
 

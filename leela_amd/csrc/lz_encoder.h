@@ -18,4 +18,11 @@ hipError_t launch_encode(const EncRecord *d_rec, int n, int symmetry,
                          uint8_t *d_policy, uint8_t *d_value,
                          hipStream_t stream);
 
+// Light playouts (playout.cpp's policy) of n boards, nplay each, playout j
+// of board p seeded playout_seed(seed0 + p, j): d_winrate[p] = mc_winrate
+// of the board for its side to move. d_half_wins (n entries) is scratch.
+hipError_t launch_playouts(const EncRecord *d_rec, int npos, int nplay,
+                           uint32_t seed0, uint32_t *d_half_wins,
+                           float *d_winrate, hipStream_t stream);
+
 }  // namespace lz

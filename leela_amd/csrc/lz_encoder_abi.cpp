@@ -32,6 +32,7 @@ struct lz_encoder {
         bool used = false;
     } slot[2];
     int next = 0;
+    uint32_t *half_wins = nullptr;  // playout outcome counters, max_batch
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -69,6 +70,11 @@ int lz_encoder_create(lz_encoder **out, int device, int max_batch) {
             return LZ_ERR_NOMEM;
         }
     }
+    if (hipMalloc(reinterpret_cast<void **>(&e->half_wins),
+                  sizeof(uint32_t) * (size_t)max_batch) != hipSuccess) {
+        lz_encoder_destroy(e);
+        return LZ_ERR_NOMEM;
+    }
     *out = e;
     return LZ_OK;
 }
@@ -84,6 +90,7 @@ void lz_encoder_destroy(lz_encoder *e) {
         if (s.dev) (void)hipFree(s.dev);
         if (s.host) (void)hipHostFree(s.host);
     }
+    if (e->half_wins) (void)hipFree(e->half_wins);
     delete e;
 }
 
@@ -95,20 +102,12 @@ const char *lz_encoder_last_error(const lz_encoder *e) {
 
 namespace {
 
-int encode(lz_encoder *e, const lz_position *const *pos, int n, int symmetry,
-           uint8_t *d_policy, uint8_t *d_value, void *stream) {
-    if (n < 0 || n > e->max_batch)
-        return e->fail(LZ_ERR_ARG, "batch %d outside 0..%d", n, e->max_batch);
-    if (symmetry < 0 || symmetry > 7)
-        return e->fail(LZ_ERR_ARG, "symmetry %d", symmetry);
-    if (n == 0) return LZ_OK;
-    if (!pos || (!d_policy && !d_value))
-        return e->fail(LZ_ERR_ARG, "null buffer");
-    for (int i = 0; i < n; i++)
-        if (!pos[i]) return e->fail(LZ_ERR_ARG, "position %d is NULL", i);
+// Packs n boards into the next record slot and sends them on s. n > 0,
+// pos checked by the caller.
+int send_records(lz_encoder *e, const lz_position *const *pos, int n,
+                 hipStream_t s, lz_encoder::Slot **out) {
     if (hipSetDevice(e->device) != hipSuccess)
         return e->fail(LZ_ERR_HIP, "hipSetDevice failed");
-    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: default stream
     lz_encoder::Slot &sl = e->slot[e->next];
     e->next ^= 1;
     if (sl.used) {
@@ -123,19 +122,70 @@ int encode(lz_encoder *e, const lz_position *const *pos, int n, int symmetry,
         const int S = pos[i]->st.board.side();
         off += (int64_t)lz::kPlanes * S * S;
     }
-    hipError_t err = hipMemcpyAsync(sl.dev, sl.host,
-                                    sizeof(lz::EncRecord) * (size_t)n,
-                                    hipMemcpyHostToDevice, s);
+    const hipError_t err = hipMemcpyAsync(sl.dev, sl.host,
+                                          sizeof(lz::EncRecord) * (size_t)n,
+                                          hipMemcpyHostToDevice, s);
     if (err != hipSuccess)
         return e->fail(LZ_ERR_HIP, "record copy: %s", hipGetErrorString(err));
-    err = lz::launch_encode(sl.dev, n, symmetry, d_policy, d_value, s);
-    if (err != hipSuccess)
-        return e->fail(LZ_ERR_HIP, "encoder launch: %s", hipGetErrorString(err));
-    err = hipEventRecord(sl.done, s);
+    *out = &sl;
+    return LZ_OK;
+}
+
+// The slot is free again once the work enqueued after its copy is done.
+int release_slot(lz_encoder *e, lz_encoder::Slot *sl, hipStream_t s) {
+    const hipError_t err = hipEventRecord(sl->done, s);
     if (err != hipSuccess)
         return e->fail(LZ_ERR_HIP, "encoder event: %s", hipGetErrorString(err));
-    sl.used = true;
+    sl->used = true;
     return LZ_OK;
+}
+
+int check_positions(lz_encoder *e, const lz_position *const *pos, int n) {
+    if (!pos) return e->fail(LZ_ERR_ARG, "null positions");
+    for (int i = 0; i < n; i++)
+        if (!pos[i]) return e->fail(LZ_ERR_ARG, "position %d is NULL", i);
+    return LZ_OK;
+}
+
+int encode(lz_encoder *e, const lz_position *const *pos, int n, int symmetry,
+           uint8_t *d_policy, uint8_t *d_value, void *stream) {
+    if (n < 0 || n > e->max_batch)
+        return e->fail(LZ_ERR_ARG, "batch %d outside 0..%d", n, e->max_batch);
+    if (symmetry < 0 || symmetry > 7)
+        return e->fail(LZ_ERR_ARG, "symmetry %d", symmetry);
+    if (n == 0) return LZ_OK;
+    if (!d_policy && !d_value) return e->fail(LZ_ERR_ARG, "null buffer");
+    int rc = check_positions(e, pos, n);
+    if (rc != LZ_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: default stream
+    lz_encoder::Slot *sl = nullptr;
+    rc = send_records(e, pos, n, s, &sl);
+    if (rc != LZ_OK) return rc;
+    const hipError_t err =
+        lz::launch_encode(sl->dev, n, symmetry, d_policy, d_value, s);
+    if (err != hipSuccess)
+        return e->fail(LZ_ERR_HIP, "encoder launch: %s", hipGetErrorString(err));
+    return release_slot(e, sl, s);
+}
+
+int playouts(lz_encoder *e, const lz_position *const *pos, int n, int nplay,
+             uint32_t seed0, float *d_winrate, void *stream) {
+    if (n < 0 || n > e->max_batch)
+        return e->fail(LZ_ERR_ARG, "batch %d outside 0..%d", n, e->max_batch);
+    if (nplay < 0) return e->fail(LZ_ERR_ARG, "playouts %d", nplay);
+    if (n == 0) return LZ_OK;
+    if (!d_winrate) return e->fail(LZ_ERR_ARG, "null buffer");
+    int rc = check_positions(e, pos, n);
+    if (rc != LZ_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: default stream
+    lz_encoder::Slot *sl = nullptr;
+    rc = send_records(e, pos, n, s, &sl);
+    if (rc != LZ_OK) return rc;
+    const hipError_t err = lz::launch_playouts(sl->dev, n, nplay, seed0,
+                                               e->half_wins, d_winrate, s);
+    if (err != hipSuccess)
+        return e->fail(LZ_ERR_HIP, "playout launch: %s", hipGetErrorString(err));
+    return release_slot(e, sl, s);
 }
 
 }  // namespace
@@ -151,6 +201,14 @@ int lz_encode_device(lz_encoder *e, const lz_position *const *pos, int n,
     if (n > 0 && !d_planes) return e->fail(LZ_ERR_ARG, "null buffer");
     return encode(e, pos, n, symmetry, net == LZ_NET_POLICY ? d_planes : nullptr,
                   net == LZ_NET_VALUE ? d_planes : nullptr, stream);
+}
+
+int lz_playout_winrates_device(lz_encoder *e, const lz_position *const *pos,
+                               int n, int playouts_per, uint32_t seed0,
+                               float *d_winrates, void *stream) {
+    if (!e) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(e->mu);
+    return playouts(e, pos, n, playouts_per, seed0, d_winrates, stream);
 }
 
 int lz_encode_device_both(lz_encoder *e, const lz_position *const *pos, int n,

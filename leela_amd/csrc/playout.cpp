@@ -56,12 +56,22 @@ float playout(GameState &st, Rng &rng) {
     return final_mc_score(b, st.komi);
 }
 
+uint32_t playout_seed(uint32_t seed, uint32_t j) {
+    uint32_t h = seed ^ (j * 0x9E3779B9u);
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
 float mc_winrate(const GameState &st, int n, uint32_t seed) {
-    Rng rng(seed);
-    rng.next();
-    rng.next();
     float bwins = 0.0f;
     for (int i = 0; i < n; i++) {
+        Rng rng(playout_seed(seed, (uint32_t)i));
+        rng.next();
+        rng.next();
         GameState tmp = st;
         const float score = playout(tmp, rng);
         if (score == 0.0f)

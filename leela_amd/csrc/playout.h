@@ -25,8 +25,14 @@ float final_mc_score(const Board &b, float komi);
 // komi.
 float playout(GameState &st, Rng &rng);
 
+// Seed of playout j of a position whose playouts are seeded `seed`: the
+// MurmurHash3 32-bit finalizer (fmix32) of seed ^ (j * 0x9E3779B9), so
+// every playout has its own stream and playouts can run in parallel.
+uint32_t playout_seed(uint32_t seed, uint32_t j);
+
 // Playout::mc_owner (Playout.cpp:174-217): mean outcome of n playouts for
-// the side to move (win 1, tie 0.5, loss 0), streams from `seed`.
+// the side to move (win 1, tie 0.5, loss 0); playout j draws from
+// Rng(playout_seed(seed, j)) with its first two outputs discarded.
 float mc_winrate(const GameState &st, int n, uint32_t seed);
 
 }  // namespace go

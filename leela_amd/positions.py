@@ -58,6 +58,8 @@ def _lib():
                                        C.c_int, vp, vp]
         L.lz_encode_device_both.argtypes = [vp, C.POINTER(vp), C.c_int,
                                             C.c_int, vp, vp, vp]
+        L.lz_playout_winrates_device.argtypes = [vp, C.POINTER(vp), C.c_int,
+                                                 C.c_int, C.c_uint32, vp, vp]
         L._positions_bound = True
     return L
 
@@ -234,6 +236,31 @@ class DeviceEncoder:
             raise LeelaError(rc, self._L.lz_encoder_last_error(self._h)
                              .decode())
 
+    def playouts_into(self, positions, playouts: int, seed0: int,
+                      d_winrates: int, stream: int = 0):
+        """Enqueue the positions' playout win rates (lz_playout_winrate on
+        the GPU, playouts of position i seeded seed0 + i) into device
+        address d_winrates (float per position)."""
+        arr, n = _as_array(positions)
+        rc = self._L.lz_playout_winrates_device(self._h, arr, n, playouts,
+                                                seed0 & 0xFFFFFFFF,
+                                                C.c_void_p(d_winrates),
+                                                C.c_void_p(stream))
+        if rc != LZ_OK:
+            raise LeelaError(rc, self._L.lz_encoder_last_error(self._h)
+                             .decode())
+
+    def playout_winrates(self, positions, playouts: int,
+                         seed0: int) -> np.ndarray:
+        """Win rates as a numpy array (synchronous)."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        out = torch.empty(max(len(positions), 1), dtype=torch.float32,
+                          device=dev)
+        self.playouts_into(positions, playouts, seed0, out.data_ptr(),
+                           torch.cuda.current_stream(dev).cuda_stream)
+        return out[:len(positions)].cpu().numpy()
+
     def encode(self, positions, net: int = NET_POLICY, symmetry: int = 0):
         """Planes as a uint8 CUDA tensor: [n, 32, S*S] for one side, flat
         for mixed sides."""
@@ -264,6 +291,38 @@ def value_labels(n: int, seed0: int, playouts: int, playout_seed: int,
         n, side, seed0, max_moves, playouts, playout_seed, threads,
         pl.ctypes.data_as(C.POINTER(C.c_uint8)) if planes else None,
         w.ctypes.data_as(C.POINTER(C.c_float))), "value_labels")
+    return pl, w
+
+
+def value_labels_device(n: int, seed0: int, playouts: int, playout_seed: int,
+                        side: int = 19, max_moves: int = MAX_MOVES_19,
+                        planes: bool = True, device: int = 0,
+                        batch: int = 4096):
+    """value_labels with the playouts (and value planes) computed on the
+    GPU: lz_playout_winrates_device and lz_encode_device over batches of
+    generator positions. Equal to value_labels bit for bit."""
+    import torch
+    enc = DeviceEncoder(device, max_batch=min(batch, max(n, 1)))
+    dev = torch.device("cuda", device)
+    pl = np.zeros((n, 32, side * side), np.uint8) if planes else None
+    w = np.zeros(n, np.float32)
+    try:
+        for a in range(0, n, enc.max_batch):
+            b = min(n, a + enc.max_batch)
+            pos = PositionBatch([Position.random(seed0 + i, side, max_moves)
+                                 for i in range(a, b)])
+            d_w = torch.empty(b - a, dtype=torch.float32, device=dev)
+            st = torch.cuda.current_stream(dev).cuda_stream
+            enc.playouts_into(pos, playouts, playout_seed + a, d_w.data_ptr(),
+                              st)
+            if planes:
+                d_pl = torch.empty((b - a, 32, side * side), dtype=torch.uint8,
+                                   device=dev)
+                enc.encode_into(pos, d_pl.data_ptr(), NET_VALUE, 0, st)
+                pl[a:b] = d_pl.cpu().numpy()
+            w[a:b] = d_w.cpu().numpy()
+    finally:
+        enc.close()
     return pl, w
 
 

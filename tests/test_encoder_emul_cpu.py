@@ -1,6 +1,7 @@
 """The device feature encoder's kernel source (leela_amd/csrc/encoder.hip)
-run on the CPU, before any GPU run, against the reference's own board
-queries and the host encoder.
+and the playout kernel's (playout.hip) run on the CPU, before any GPU run,
+against the reference's own board queries, the host encoder and the host
+playouts.
 
 The kernel is compiled by g++ against tests/wave_emul/hip/hip_runtime.h,
 which runs each workgroup as 64 host threads (one wavefront): __shared__
@@ -30,8 +31,10 @@ EMUL = Path(__file__).parent / "wave_emul"
 CSRC = ROOT / "leela_amd" / "csrc"
 OUT = ROOT / "_build" / "wave_emul" / "libencoder_emul.so"
 GOLD = Path(__file__).parent / "golden" / "board_golden.npz"
-SOURCES = [CSRC / "encoder.hip", EMUL / "driver.cpp", CSRC / "go_board.cpp",
-           CSRC / "input_planes.cpp", CSRC / "lz_enc_record.cpp"]
+SOURCES = [CSRC / "encoder.hip", CSRC / "playout.hip", EMUL / "driver.cpp",
+           CSRC / "go_board.cpp", CSRC / "input_planes.cpp",
+           CSRC / "lz_enc_record.cpp", CSRC / "playout.cpp"]
+HIP_SOURCES = 2
 
 
 def _build():
@@ -45,8 +48,9 @@ def _build():
     OUT.parent.mkdir(parents=True, exist_ok=True)
     cmd = [cxx, "-std=c++20", "-O2", "-pthread", "-fPIC", "-shared",
            "-Wno-unknown-pragmas", f"-I{EMUL}", f"-I{CSRC}",
-           f"-I{ROOT / 'include'}", "-x", "c++", str(SOURCES[0]), "-x",
-           "none", *map(str, SOURCES[1:]), "-o", str(OUT)]
+           f"-I{ROOT / 'include'}", "-x", "c++",
+           *map(str, SOURCES[:HIP_SOURCES]), "-x", "none",
+           *map(str, SOURCES[HIP_SOURCES:]), "-o", str(OUT)]
     subprocess.run(cmd, check=True, capture_output=True)
     return OUT
 
@@ -60,6 +64,9 @@ def emul():
                                     u8p]
     L.emul_encode_moves_both.argtypes = [C.c_int, C.c_int, i32p, C.c_int,
                                          u8p, u8p]
+    f32p = C.POINTER(C.c_float)
+    L.emul_playouts.argtypes = [C.c_int, i32p, C.POINTER(C.c_uint32), i32p,
+                                C.c_int, C.c_uint32, f32p, f32p]
     L.emul_encode_generated.argtypes = [C.c_int, i32p,
                                         C.POINTER(C.c_uint32), i32p, C.c_int,
                                         C.c_int, u8p, u8p]
@@ -142,3 +149,20 @@ def test_kernel_matches_host_encoder_ragged(emul, net, sym):
         sz = 32 * int(s) * int(s)
         assert np.array_equal(out[off:off + sz], ref[off:off + sz]), (i, s)
         off += sz
+
+
+def test_playout_kernel_matches_host_playouts(emul):
+    """playout.hip's kernel source against the host's mc_winrate, bit for
+    bit: generator positions of three sides (early, middle and end of
+    game), 8 playouts each."""
+    n = 6
+    sides = np.array([(19, 13, 9)[i % 3] for i in range(n)], np.int32)
+    seeds = np.arange(40, 40 + n, dtype=np.uint32)
+    mm = np.array([(250, 101, 48)[i % 3] for i in range(n)], np.int32)
+    out = np.full(n, -1.0, np.float32)
+    ref = np.zeros(n, np.float32)
+    emul.emul_playouts(n, _p(sides, C.c_int32), _p(seeds, C.c_uint32),
+                       _p(mm, C.c_int32), 8, 777, _p(out, C.c_float),
+                       _p(ref, C.c_float))
+    assert np.array_equal(out, ref), (out, ref)
+    assert len(set(ref.tolist())) > 3  # the outcomes differ by position

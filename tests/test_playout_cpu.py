@@ -102,13 +102,14 @@ def test_playouts_deterministic_and_bounded():
 
 def test_playouts_see_a_won_position():
     # Black owns most of a 9x9 board (a wall on column 6, nothing else):
-    # black wins nearly every playout under komi 7.5.
+    # black wins most playouts under komi 7.5 (0.876 over 4096 playouts;
+    # random play breaks the wall now and then).
     p = Position(9, 7.5)
     for y in range(9):
         p.play(p.vertex(6, y))
         p.play(PASS)
     w = p.playout_winrate(64, 3)             # black to move
-    assert w > 0.9
+    assert w > 0.75
 
 
 def test_value_labels_match_single_entry_points():

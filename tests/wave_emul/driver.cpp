@@ -9,6 +9,7 @@
 #include "go_board.h"
 #include "input_planes.h"
 #include "lz_encoder.h"
+#include "playout.h"
 
 using lz::go::GameState;
 
@@ -65,6 +66,24 @@ int emul_encode_generated(int n, const int32_t *sides, const uint32_t *seeds,
         off += (int64_t)lz::kPlanes * S * S;
     }
     run(b, net, sym, out);
+    return 0;
+}
+
+// Playout win rates of generator positions (sides[i], seeds[i],
+// max_moves[i]), nplay playouts each, position i seeded seed0 + i: out
+// from the kernel, ref from the host's mc_winrate.
+int emul_playouts(int n, const int32_t *sides, const uint32_t *seeds,
+                  const int32_t *max_moves, int nplay, uint32_t seed0,
+                  float *out, float *ref) {
+    std::vector<GameState> b(n);
+    std::vector<lz::EncRecord> rec(n);
+    for (int i = 0; i < n; i++) {
+        lz::go::random_position(b[i], sides[i], seeds[i], max_moves[i]);
+        lz::pack_record(b[i], 0, &rec[i]);
+        ref[i] = lz::go::mc_winrate(b[i], nplay, seed0 + (uint32_t)i);
+    }
+    std::vector<uint32_t> half(n);
+    lz::launch_playouts(rec.data(), n, nplay, seed0, half.data(), out, nullptr);
     return 0;
 }
 

@@ -108,8 +108,8 @@ inline unsigned long long __builtin_amdgcn_ballot_w64(bool p) {
     return m;
 }
 
-// DPP controls the kernel uses: row_shl:1, row_shr:1, quad_perm [1,0,3,2]
-// and [2,3,0,1], row_half_mirror.
+// DPP controls the kernels use: row_shl:1, row_shr:1/2/4, quad_perm
+// [1,0,3,2] and [2,3,0,1], row_half_mirror.
 inline int __builtin_amdgcn_update_dpp(int, int src, int ctrl, int, int, bool) {
     const int lane = (int)threadIdx.x, gl = lane & 7;
     wave_emul::Barrier &b = wave_emul::group_bar[lane >> 3];
@@ -118,7 +118,9 @@ inline int __builtin_amdgcn_update_dpp(int, int src, int ctrl, int, int, bool) {
     int from = -1;
     switch (ctrl) {
         case 0x101: from = gl == 7 ? -1 : lane + 1; break;
-        case 0x111: from = gl == 0 ? -1 : lane - 1; break;
+        case 0x111: from = gl < 1 ? -1 : lane - 1; break;
+        case 0x112: from = gl < 2 ? -1 : lane - 2; break;
+        case 0x114: from = gl < 4 ? -1 : lane - 4; break;
         case 0xB1: from = lane ^ 1; break;
         case 0x4E: from = lane ^ 2; break;
         case 0x141: from = (lane & ~7) | (7 - gl); break;
@@ -127,6 +129,13 @@ inline int __builtin_amdgcn_update_dpp(int, int src, int ctrl, int, int, bool) {
     const int r = from < 0 ? 0 : wave_emul::gslot[from];
     wave_emul::arrive(b);
     return r;
+}
+
+// The kernels order a group's LDS writes before the group's next reads
+// with a wave barrier (on the hardware the wavefront executes in order);
+// here the group's lanes meet.
+inline void __builtin_amdgcn_wave_barrier() {
+    wave_emul::arrive(wave_emul::group_bar[threadIdx.x >> 3]);
 }
 
 inline unsigned long long atomicOr(unsigned long long *p, unsigned long long v) {
@@ -141,6 +150,11 @@ inline unsigned atomicOr(unsigned *p, unsigned v) {
 }
 inline int atomicAdd(int *p, int v) {
     const int o = *p;
+    *p = o + v;
+    return o;
+}
+inline unsigned atomicAdd(unsigned *p, unsigned v) {
+    const unsigned o = *p;
     *p = o + v;
     return o;
 }
