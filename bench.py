@@ -305,6 +305,25 @@ def main() -> int:
         e2e_step(k)
     torch.cuda.synchronize(dev)
     e2e_pos_s = args.steps * B / (time.perf_counter() - t0)
+    # Playouts of the surrogate's teacher (lz_playout_winrates_device vs
+    # lz_make_value_labels on the host threads), 64 per position.
+    po_n, po_k = 2048, 64
+    po_pos = positions.PositionBatch(
+        [positions.Position.random(10000 + i) for i in range(po_n)])
+    d_po = torch.empty(po_n, dtype=torch.float32, device=dev)
+    pdenc = positions.DeviceEncoder(0, max_batch=po_n)
+    pdenc.playouts_into(po_pos, po_k, 5, d_po.data_ptr(), stream)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    pdenc.playouts_into(po_pos, po_k, 5, d_po.data_ptr(), stream)
+    torch.cuda.synchronize(dev)
+    gpu_po_s = po_n * po_k / (time.perf_counter() - t0)
+    pdenc.close()
+    host_n = 128
+    t0 = time.perf_counter()
+    positions.value_labels(host_n, 10000, po_k, 5, threads=enc_threads,
+                           planes=False)
+    host_po_s = host_n * po_k / (time.perf_counter() - t0)
     denc.close()
 
     # Dominant kernel, timed per launch with HIP events on its stream.
@@ -374,6 +393,14 @@ def main() -> int:
                           "lz_encode_device_both (own stream, one batch "
                           "ahead) -> policy + value forward, one host "
                           "thread",
+            "playouts": {
+                "device_playouts_per_s": round(gpu_po_s, 1),
+                "host_playouts_per_s": round(host_po_s, 1),
+                "host_threads": enc_threads,
+                "sample": f"{po_n} generator positions x {po_k} light "
+                          f"playouts on the device, {host_n} x {po_k} on "
+                          f"the host; equal results bit for bit",
+            },
         },
         "roofline": {
             "bound": "mfma",

@@ -4,11 +4,11 @@
 //
 // Each group holds its board as bitboards spread over its lanes
 // (lz_wbits.h) and plays until four passes in a row or move number
-// 2 * side^2, exactly as the host playout: per move the empty points are
-// listed in vertex order (LDS, one list per group), a draw
-// randuint32(n) picks one, and a point that is the ko square, an own eye
-// or suicide is replaced by the list's last entry and the draw repeated;
-// no candidate left is a pass. The move captures opponent strings left
+// 2 * side^2, exactly as the host playout: per move the empty points in
+// vertex order are the candidates, a draw randuint32(n) picks one, and a
+// point that is the ko square, an own eye or suicide is replaced by the
+// last candidate and the draw repeated (the list is built in LDS, one per
+// group, only once a draw is rejected); no candidate left is a pass. The move captures opponent strings left
 // without liberties, and a single captured stone after a move into an
 // opponent eye is the new ko square (FullBoard::update_board,
 // FullBoard.cpp:130-202). The outcome (black win 2, tie 1, loss 0 in
@@ -108,20 +108,60 @@ __device__ __forceinline__ uint64_t p_empties(const PBoard &b, const Grp &g) {
     return g.on & ~(b.c[0] | b.c[1]);
 }
 
+// Whether the part of `mask` connected to s touches `target`: the fill of
+// w_fill, stopped at the first contact (most strings next to a move have
+// a liberty close by).
+__device__ bool w_reaches(uint64_t s, uint64_t mask, uint64_t target,
+                          const Grp &g) {
+    for (;;) {
+        const uint64_t n = w_nbrs(s, g);
+        if (w_any(n & target, g)) return true;
+        const uint64_t t = s | (n & mask);
+        if (!w_any(t ^ s, g)) return false;
+        s = t;
+    }
+}
+
+// The k-th (0-based) member of s in vertex order: the lane whose prefix
+// count covers k selects within its word by halving.
+__device__ int w_select(uint64_t s, int k, const Grp &g) {
+    const int cnt = __popcll(s);
+    const int pre = gscan_excl(cnt, g.gl);
+    int v = 0;
+    if (k >= pre && k < pre + cnt) {
+        int r = k - pre, base = 0;
+        uint64_t w = s;
+#pragma unroll
+        for (int h = 32; h >= 1; h >>= 1) {
+            const uint64_t lo = w & ((1ull << h) - 1);
+            const int c = __popcll(lo);
+            if (r >= c) {
+                r -= c;
+                base += h;
+                w >>= h;
+            } else {
+                w = lo;
+            }
+        }
+        v = 64 * g.gl + base;
+    }
+    return gsum(v);
+}
+
 // FastBoard::is_suicide (FastBoard.cpp:191-240) for an empty v: no empty
 // neighbour, every own neighbour string has v as its only liberty, and no
-// opponent neighbour string is in atari.
+// opponent neighbour string is in atari. (v is a liberty of each
+// neighbour string, so both tests ask whether the string has another.)
 __device__ bool is_suicide(const PBoard &b, const Grp &g, int v, int col, int f) {
 #pragma unroll
     for (int k = 0; k < 4; k++)
         if (fld(f, k) == kEmpty) return false;
-    const uint64_t E = p_empties(b, g);
+    const uint64_t other = p_empties(b, g) & ~w_point(g, v);
     for (int k = 0; k < 4; k++) {
         const int s = fld(f, k);
         if (s != kBlack && s != kWhite) continue;
-        const uint64_t t = w_fill(w_point(g, v + g.d[k]), b.c[s], g);
-        const int l = w_count(w_nbrs(t, g) & E);
-        if (s == col ? l != 1 : l <= 1) return false;
+        const bool more = w_reaches(w_point(g, v + g.d[k]), b.c[s], other, g);
+        if (s == col ? more : !more) return false;
     }
     return true;
 }
@@ -141,8 +181,8 @@ __device__ int play_move(PBoard &b, const Grp &g, int col, int v, int f) {
         if (fld(f, k) != (col ^ 1)) continue;
         const int a = v + g.d[k];
         if (!w_test(b.c[col ^ 1], g, a)) continue;  // taken with an earlier one
-        const uint64_t t = w_fill(w_point(g, a), b.c[col ^ 1], g);
-        if (!w_any(w_nbrs(t, g) & E, g)) {
+        if (!w_reaches(w_point(g, a), b.c[col ^ 1], E, g)) {
+            const uint64_t t = w_fill(w_point(g, a), b.c[col ^ 1], g);
             captured += w_count(t);
             b.c[col ^ 1] &= ~t;
             E |= t;
@@ -190,26 +230,32 @@ __global__ __launch_bounds__(64) void playout_kernel(const EncRecord *recs,
     uint16_t *list = s_list[grp];
 
     while (passes < 4 && movenum < maxmoves) {
-        // The empty points in vertex order: each lane writes its word's.
+        // The candidates are the empty points in vertex order. The first
+        // draw picks from them directly (select); only after a rejection
+        // is the list built in LDS (each lane writes its word's points),
+        // and then rejected entries are replaced by the last one.
         uint64_t e = p_empties(b, g);
-        const int cnt = __popcll(e);
-        int pos = gscan_excl(cnt, g.gl);
-        int n = gsum(cnt);
-        while (e) {
-            list[pos++] = (uint16_t)(64 * g.gl + __builtin_ctzll(e));
-            e &= e - 1;
-        }
-        __builtin_amdgcn_wave_barrier();
+        int n = w_count(e);
         int move = kPassMove, f = 0;
+        bool listed = false;
         while (n > 0) {
             const int i = (int)rng.randuint32((uint32_t)n);
-            const int v = list[i];
+            const int v = listed ? (int)list[i] : w_select(e, i, g);
             if (v != ko) {
                 f = nbr8(b, g, v);
                 if (!is_eye(f, tm) && !is_suicide(b, g, v, tm, f)) {
                     move = v;
                     break;
                 }
+            }
+            if (!listed) {
+                const int cnt = __popcll(e);
+                int pos = gscan_excl(cnt, g.gl);
+                while (e) {
+                    list[pos++] = (uint16_t)(64 * g.gl + __builtin_ctzll(e));
+                    e &= e - 1;
+                }
+                listed = true;
             }
             __builtin_amdgcn_wave_barrier();
             list[i] = list[--n];  // every lane of the group writes the same

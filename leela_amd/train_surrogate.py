@@ -8,7 +8,8 @@ Training positions are generator positions from seeds seed0..seed0+n-1
 (move counts uniform in 0..250); their labels are the mean outcome of
 `--playouts` playouts each (playouts of position i seeded
 playout_seed0 + i), read from `--labels` (a float32 .npy of n win rates)
-when given, generated otherwise. Training is Adam on minibatches of
+when given, generated otherwise (GPU playouts; `--host-labels` for the
+host's, which are equal). Training is Adam on minibatches of
 `--batch` (learning rate ramped over `--warmup` steps), with a checkpoint
 every `--ckpt-every` steps and at the end.
 
@@ -51,6 +52,9 @@ def main(argv=None) -> int:
     ap.add_argument("--infer-batch", type=int, default=4096)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--log-every", type=int, default=200)
+    ap.add_argument("--host-labels", action="store_true",
+                    help="label with host playouts (default: GPU playouts, "
+                         "the same values)")
     args = ap.parse_args(argv)
 
     import torch
@@ -66,10 +70,13 @@ def main(argv=None) -> int:
         planes = positions.make_batch(args.n, seed0=args.seed0,
                                       net=positions.NET_VALUE,
                                       threads=args.threads)
-    else:
+    elif args.host_labels:
         planes, labels = positions.value_labels(
             args.n, args.seed0, args.playouts, args.playout_seed0,
             threads=args.threads)
+    else:
+        planes, labels = positions.value_labels_device(
+            args.n, args.seed0, args.playouts, args.playout_seed0)
         if args.labels:
             np.save(args.labels, labels)
     t_data = time.perf_counter() - t0
