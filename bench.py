@@ -326,6 +326,18 @@ def main() -> int:
     host_po_s = host_n * po_k / (time.perf_counter() - t0)
     denc.close()
 
+    # Network::benchmark restated (the netbench command): 16 threads of
+    # synchronous get_scored_moves / get_value calls (RANDOM_ROTATION, host
+    # feature extraction per call), batched by the leaf queue.
+    from leela_amd.network import Network
+    nb_threads = 16
+    be.queue_start(p, max_batch=nb_threads, max_wait_us=200)
+    try:
+        nb_policy, nb_value = Network(be, p).benchmark(
+            positions.Position.random(1), nb_threads)
+    finally:
+        be.queue_stop()
+
     # Dominant kernel, timed per launch with HIP events on its stream.
     kern_ms = []
     settle(p)
@@ -393,6 +405,15 @@ def main() -> int:
                           "lz_encode_device_both (own stream, one batch "
                           "ahead) -> policy + value forward, one host "
                           "thread",
+            "netbench": {
+                "policy_per_s": round(nb_policy, 1),
+                "value_per_s": round(nb_value, 1),
+                "threads": nb_threads,
+                "what": "Network::benchmark (Network.cpp:147-199): 2000 "
+                        "policy then 10000 value evaluations, "
+                        "RANDOM_ROTATION, synchronous calls from 16 "
+                        "threads through the leaf queue (max batch 16)",
+            },
             "playouts": {
                 "device_playouts_per_s": round(gpu_po_s, 1),
                 "host_playouts_per_s": round(host_po_s, 1),

@@ -304,6 +304,16 @@ int lz_network_scored_moves(lz_ctx *ctx, int precision,
 int lz_network_value(lz_ctx *ctx, int precision, const lz_position *pos,
                      int ensemble, float *value);
 
+/* Network::benchmark (Network.cpp:147-199), the netbench command: threads
+ * threads, each on a copy of pos (19x19), run ceil(2000/threads)
+ * lz_network_scored_moves and then ceil(10000/threads) lz_network_value
+ * calls, RANDOM_ROTATION; *policy_ps = 2000 and *value_ps = 10000 over the
+ * elapsed seconds. The synchronous facade calls go through the leaf queue
+ * when one runs at `precision` (lz_queue_start), so the threads' calls are
+ * batched together; otherwise each is one lz_forward. */
+int lz_network_benchmark(lz_ctx *ctx, int precision, const lz_position *pos,
+                         int threads, float *policy_ps, float *value_ps);
+
 /* Network::async_scored_moves (Network.cpp:536-587) with DIRECT rotation,
  * through the leaf queue (lz_queue_start first; the queue's precision
  * applies). `done` gets the Netresult on the dispatcher thread, as

@@ -46,4 +46,6 @@ int forward_locked(lz_ctx *ctx, int net, int precision, int n,
                    hipStream_t stream, LaunchTimer *timer = nullptr);
 // Stops and frees the leaf queue of ctx, if any (lz_queue.cpp).
 void destroy_queue(lz_ctx *ctx);
+// The running leaf queue's precision, -1 when none runs.
+int queue_precision(const lz_ctx *ctx);
 }  // namespace lz

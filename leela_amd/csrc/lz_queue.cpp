@@ -57,6 +57,8 @@ public:
 
     ~LeafQueue() { stop(); }
 
+    int precision() const { return precision_; }
+
     int start(std::string *msg) {
         hipError_t e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
         if (e == hipSuccess)
@@ -301,6 +303,10 @@ private:
     float *h_raw_ = nullptr, *h_out_ = nullptr;
     float *d_raw_ = nullptr, *d_out_ = nullptr;
 };
+
+int queue_precision(const lz_ctx *ctx) {
+    return ctx->queue ? ctx->queue->precision() : -1;
+}
 
 void destroy_queue(lz_ctx *ctx) {
     if (ctx->queue) {

@@ -2,13 +2,17 @@
 // the C ABI, plus its C entry points.
 #include "network.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 #include "input_planes.h"
+#include "lz_ctx.h"
 #include "lz_position.h"
 #include "lz_random.h"
 
@@ -76,6 +80,87 @@ void prune_ladders(const go::GameState &state, const uint8_t *ladder,
     }
 }
 
+namespace {
+
+// One synchronous call's rows in flight through the leaf queue.
+struct SyncCall {
+    std::mutex m;
+    std::condition_variable cv;
+    int left = 0;
+    int status = LZ_OK;
+    float *out = nullptr;
+    int width = 0;
+};
+
+struct SyncRow {
+    SyncCall *call;
+    int row;
+};
+
+void sync_done(void *user, int status, const float *raw, const float *out) {
+    (void)raw;
+    SyncRow *r = static_cast<SyncRow *>(user);
+    SyncCall *c = r->call;
+    if (status == LZ_OK)
+        std::copy(out, out + c->width, c->out + (size_t)r->row * c->width);
+    std::lock_guard<std::mutex> g(c->m);
+    if (status != LZ_OK) c->status = status;
+    if (--c->left == 0) c->cv.notify_all();
+}
+
+}  // namespace
+
+int Network::forward_rows(int net, int rows, const uint8_t *in, float *out) {
+    if (queue_precision(ctx_) != precision_)
+        return lz_forward(ctx_, net, precision_, rows, in, nullptr, out);
+    SyncCall call;
+    call.out = out;
+    call.width = net == LZ_NET_POLICY ? kP : 1;
+    std::vector<SyncRow> row(rows);
+    int submitted = 0, rc = LZ_OK;
+    {
+        std::lock_guard<std::mutex> g(call.m);
+        call.left = rows;
+    }
+    for (int r = 0; r < rows && rc == LZ_OK; r++) {
+        row[r] = {&call, r};
+        rc = lz_submit(ctx_, net, in + (size_t)r * kPos, sync_done, &row[r]);
+        if (rc == LZ_OK) submitted++;
+    }
+    std::unique_lock<std::mutex> lk(call.m);
+    call.left -= rows - submitted;  // rows never queued
+    call.cv.wait(lk, [&] { return call.left == 0; });
+    return rc != LZ_OK ? rc : call.status;
+}
+
+void Network::benchmark(const go::GameState &state, int threads,
+                        float *policy_ps, float *value_ps) {
+    const int cpus = threads < 1 ? 1 : threads;
+    auto run = [&](int amount, bool policy) {
+        const int iters = (amount + cpus - 1) / cpus;
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::thread> pool;
+        for (int i = 0; i < cpus; i++)
+            pool.emplace_back([&, iters, policy] {
+                go::GameState mystate = state;
+                Network net(ctx_, precision_);
+                for (int loop = 0; loop < iters; loop++) {
+                    if (policy)
+                        net.get_scored_moves(mystate, RANDOM_ROTATION);
+                    else
+                        net.get_value(mystate, RANDOM_ROTATION);
+                }
+            });
+        for (auto &t : pool) t.join();
+        const double s = std::chrono::duration<double>(
+                             std::chrono::steady_clock::now() - t0)
+                             .count();
+        return (float)(amount / s);
+    };
+    *policy_ps = run(2000, true);
+    *value_ps = run(10000, false);
+}
+
 Network::Netresult Network::get_scored_moves(const go::GameState &state,
                                              Ensemble ensemble,
                                              int rotation) {
@@ -98,8 +183,7 @@ Network::Netresult Network::get_scored_moves(const go::GameState &state,
     for (int r = 0; r < rows; r++)
         rotate_planes(planes.data(), rots[r], 19, in.data() + (size_t)r * kPos);
     std::vector<float> probs((size_t)rows * kP);
-    status_ = lz_forward(ctx_, LZ_NET_POLICY, precision_, rows, in.data(),
-                         nullptr, probs.data());
+    status_ = forward_rows(LZ_NET_POLICY, rows, in.data(), probs.data());
     if (status_ != LZ_OK) return result;
 
     collect_scored_moves(state, probs.data(), rots[0], result);
@@ -131,8 +215,7 @@ float Network::get_value(const go::GameState &state, Ensemble ensemble) {
         rotate_planes(planes.data(), rot, 19, in.data() + (size_t)r * kPos);
     }
     std::vector<float> win(rows);
-    status_ = lz_forward(ctx_, LZ_NET_VALUE, precision_, rows, in.data(),
-                         nullptr, win.data());
+    status_ = forward_rows(LZ_NET_VALUE, rows, in.data(), win.data());
     if (status_ != LZ_OK) return 0.5f;
     float sum = 0.0f;
     for (float w : win) sum += w;
@@ -221,6 +304,16 @@ int lz_network_value(lz_ctx *ctx, int precision, const lz_position *pos,
     *value = net.get_value(pos->st,
                            static_cast<lz::Network::Ensemble>(ensemble));
     return net.last_status();
+}
+
+int lz_network_benchmark(lz_ctx *ctx, int precision, const lz_position *pos,
+                         int threads, float *policy_ps, float *value_ps) {
+    if (!ctx || !pos || !policy_ps || !value_ps || threads < 1)
+        return LZ_ERR_ARG;
+    if (pos->st.board.side() != 19) return LZ_ERR_ARG;
+    lz::Network net(ctx, precision);
+    net.benchmark(pos->st, threads, policy_ps, value_ps);
+    return LZ_OK;
 }
 
 int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,

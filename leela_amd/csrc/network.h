@@ -60,7 +60,21 @@ public:
 
     int last_status() const { return status_; }
 
+    // Network::benchmark (Network.cpp:147-199): `threads` threads, each on
+    // its own copy of the state, run ceil(2000 / threads) policy
+    // evaluations and then ceil(10000 / threads) value evaluations, all
+    // RANDOM_ROTATION; the rates are 2000 and 10000 over the elapsed
+    // seconds, as the reference prints them.
+    void benchmark(const go::GameState &state, int threads, float *policy_ps,
+                   float *value_ps);
+
 private:
+    // Forward of `rows` planes for the synchronous calls: through the leaf
+    // queue when one runs at this precision (concurrent callers are
+    // batched together, as the reference's per-thread OpenCL queues are
+    // not), else one lz_forward.
+    int forward_rows(int net, int rows, const uint8_t *in, float *out);
+
     lz_ctx *ctx_;
     int precision_;
     int status_ = LZ_OK;

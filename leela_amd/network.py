@@ -26,6 +26,9 @@ def _lib():
                                        C.POINTER(C.c_float)]
         L.lz_network_async_scored_moves.argtypes = [vp, vp, C.c_int,
                                                     MOVES_FN, vp]
+        L.lz_network_benchmark.argtypes = [vp, C.c_int, vp, C.c_int,
+                                           C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float)]
         L._network_bound = True
     return L
 
@@ -58,6 +61,16 @@ class Network:
         if rc != LZ_OK:
             raise LeelaError(rc, "lz_network_value")
         return float(v.value)
+
+    def benchmark(self, pos, threads: int):
+        """Network::benchmark (netbench): (policy p/s, value p/s)."""
+        p, v = C.c_float(0), C.c_float(0)
+        rc = _lib().lz_network_benchmark(self._be._ctx, self.precision,
+                                         pos._p, threads, C.byref(p),
+                                         C.byref(v))
+        if rc != LZ_OK:
+            raise LeelaError(rc, "lz_network_benchmark")
+        return float(p.value), float(v.value)
 
     def async_scored_moves(self, pos, rotation: int, done, user: int = 0):
         """done: a MOVES_FN-wrapped callable (user, count, probs, vertices)

@@ -172,3 +172,54 @@ def test_async_scored_moves_through_queue(be):
     for i in range(len(poss)):
         assert np.array_equal(got[i][1], want[i][1])
         assert np.array_equal(got[i][0], want[i][0])
+
+
+def test_sync_calls_batch_through_queue(be):
+    """With the leaf queue running, the synchronous facade calls of many
+    threads are batched together and give the direct results bit for bit
+    (FP32: rows are independent of their batch)."""
+    import leela_amd
+    from leela_amd.network import DIRECT, Network
+    net = Network(be, leela_amd.PREC_FP32)
+    poss = [Position.random(s, 19, 250) for s in range(700, 748)]
+    want_p = [net.get_scored_moves(p, DIRECT, i % 8) for i, p in enumerate(poss)]
+    want_v = [net.get_value(p, DIRECT) for p in poss]
+    got_p = [None] * len(poss)
+    got_v = [None] * len(poss)
+    be.queue_start(leela_amd.PREC_FP32, max_batch=64, max_wait_us=2000)
+    try:
+        def worker(t):
+            for i in range(t, len(poss), 16):
+                got_p[i] = net.get_scored_moves(poss[i], DIRECT, i % 8)
+                got_v[i] = net.get_value(poss[i], DIRECT)
+        ths = [threading.Thread(target=worker, args=(t,)) for t in range(16)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        stats = be.queue_stats()
+    finally:
+        be.queue_stop()
+    for i in range(len(poss)):
+        assert np.array_equal(got_p[i][0], want_p[i][0])
+        assert np.array_equal(got_p[i][1], want_p[i][1])
+        assert got_v[i] == want_v[i]
+    # 96 calls arrived in fewer launches than calls: they were batched.
+    assert stats["positions"] == 2 * len(poss)
+    assert stats["batches"] < stats["positions"]
+
+
+def test_netbench(be):
+    """Network::benchmark restated: both rates positive, through the queue
+    and through direct forwards."""
+    import leela_amd
+    from leela_amd.network import Network
+    net = Network(be, leela_amd.PREC_FP16)
+    pos = Position.random(3, 19, 250)
+    direct = net.benchmark(pos, 4)
+    be.queue_start(leela_amd.PREC_FP16, max_batch=64, max_wait_us=200)
+    try:
+        queued = net.benchmark(pos, 16)
+    finally:
+        be.queue_stop()
+    assert min(direct) > 0 and min(queued) > 0
