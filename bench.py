@@ -258,38 +258,39 @@ def main() -> int:
     # Device encoder (lz_encode_device_both): both nets' planes computed on
     # the GPU from the positions (host side: one record per board, one
     # copy). End to end: positions in host memory -> planes -> policy +
-    # value outputs, the encoder on its own stream one batch ahead of the
-    # nets (two plane buffers, events between the streams), as a search
-    # would feed it.
-    denc = positions.DeviceEncoder(0, max_batch=B)
+    # value outputs on the benchmark stream, one encoder launch per E_AHEAD
+    # net batches (a launch lasts as long as its slowest ladder reading,
+    # which a bigger launch shares among more boards). The host packs the
+    # next launch's records while the device runs the previous batches.
+    # (On a second stream the encoder's waves hold CUs that the persistent
+    # net kernels wait for: measured slower than in line.)
+    E_AHEAD = 4
+    denc = positions.DeviceEncoder(0, max_batch=E_AHEAD * B)
     pbatch = positions.PositionBatch(pos_objs)
-    d_epl = [torch.empty_like(d_planes) for _ in range(2)]
-    d_evpl = [torch.empty_like(d_vplanes) for _ in range(2)]
-    enc_stream = torch.cuda.Stream(dev)
-    nn_stream = bench_stream
-    ev_enc = [torch.cuda.Event() for _ in range(2)]
-    ev_nn = [torch.cuda.Event() for _ in range(2)]
+    ebatch = positions.PositionBatch(
+        [positions.Position.random(1 + i) for i in range(E_AHEAD * B)])
+    d_epl = torch.empty((E_AHEAD * B,) + tuple(d_planes.shape[1:]),
+                        dtype=torch.uint8, device=dev)
+    d_evpl = torch.empty_like(d_epl)
 
-    def encode_step(j=0, st=stream):
-        denc.encode_both_into(pbatch, d_epl[j].data_ptr(),
-                              d_evpl[j].data_ptr(), 0, st)
+    def encode_step(batch=pbatch):
+        denc.encode_both_into(batch, d_epl.data_ptr(), d_evpl.data_ptr(), 0,
+                              stream)
 
     def e2e_step(k):
-        j = k % 2
-        enc_stream.wait_event(ev_nn[j])  # the nets are done with buffer j
-        encode_step(j, enc_stream.cuda_stream)
-        ev_enc[j].record(enc_stream)
-        nn_stream.wait_event(ev_enc[j])
-        be.forward_device(leela_amd.NET_POLICY, p, B, d_epl[j].data_ptr(),
-                          d_praw.data_ptr(), d_pout.data_ptr(), stream)
-        be.forward_device(leela_amd.NET_VALUE, p, B, d_evpl[j].data_ptr(),
-                          d_vraw.data_ptr(), d_vout.data_ptr(), stream)
-        ev_nn[j].record(nn_stream)
+        encode_step(ebatch)
+        for q in range(E_AHEAD):
+            pl = d_epl[q * B:(q + 1) * B]
+            vpl = d_evpl[q * B:(q + 1) * B]
+            be.forward_device(leela_amd.NET_POLICY, p, B, pl.data_ptr(),
+                              d_praw.data_ptr(), d_pout.data_ptr(), stream)
+            be.forward_device(leela_amd.NET_VALUE, p, B, vpl.data_ptr(),
+                              d_vraw.data_ptr(), d_vout.data_ptr(), stream)
 
     encode_step()
     torch.cuda.synchronize(dev)
-    denc_exact = bool(torch.equal(d_epl[0], d_planes) and
-                      torch.equal(d_evpl[0], d_vplanes))
+    denc_exact = bool(torch.equal(d_epl[:B], d_planes) and
+                      torch.equal(d_evpl[:B], d_vplanes))
     reps = max(3, args.steps // 2)
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -300,11 +301,13 @@ def main() -> int:
     for k in range(2):
         e2e_step(k)
     torch.cuda.synchronize(dev)
+    e2e_launches = max(2, args.steps // E_AHEAD)
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(e2e_launches):
         e2e_step(k)
     torch.cuda.synchronize(dev)
-    e2e_pos_s = args.steps * B / (time.perf_counter() - t0)
+    e2e_pos_s = e2e_launches * E_AHEAD * B / (time.perf_counter() - t0)
+
     # Playouts of the surrogate's teacher (lz_playout_winrates_device vs
     # lz_make_value_labels on the host threads), 64 per position.
     po_n, po_k = 2048, 64
@@ -402,9 +405,9 @@ def main() -> int:
             "device_encoder_planes_equal_host": denc_exact,
             "end_to_end_positions_per_s": round(e2e_pos_s, 1),
             "end_to_end": "positions in host memory -> "
-                          "lz_encode_device_both (own stream, one batch "
-                          "ahead) -> policy + value forward, one host "
-                          "thread",
+                          "lz_encode_device_both (one launch per 4 net "
+                          "batches) -> policy + value forward per 1024, one "
+                          "stream, one host thread",
             "netbench": {
                 "policy_per_s": round(nb_policy, 1),
                 "value_per_s": round(nb_value, 1),
