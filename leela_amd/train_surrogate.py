@@ -61,6 +61,14 @@ def main(argv=None) -> int:
     from . import positions
     from .surrogate import Surrogate, train
 
+    def make_labels(n, seed0, playouts, playout_seed0, planes=True):
+        # GPU playouts unless asked for the host's (equal values).
+        if args.host_labels:
+            return positions.value_labels(n, seed0, playouts, playout_seed0,
+                                          threads=args.threads, planes=planes)
+        return positions.value_labels_device(n, seed0, playouts,
+                                             playout_seed0, planes=planes)
+
     t0 = time.perf_counter()
     if args.labels and os.path.exists(args.labels):
         labels = np.load(args.labels, allow_pickle=False).astype(np.float32)
@@ -70,13 +78,9 @@ def main(argv=None) -> int:
         planes = positions.make_batch(args.n, seed0=args.seed0,
                                       net=positions.NET_VALUE,
                                       threads=args.threads)
-    elif args.host_labels:
-        planes, labels = positions.value_labels(
-            args.n, args.seed0, args.playouts, args.playout_seed0,
-            threads=args.threads)
     else:
-        planes, labels = positions.value_labels_device(
-            args.n, args.seed0, args.playouts, args.playout_seed0)
+        planes, labels = make_labels(args.n, args.seed0, args.playouts,
+                                     args.playout_seed0)
         if args.labels:
             np.save(args.labels, labels)
     t_data = time.perf_counter() - t0
@@ -106,22 +110,20 @@ def main(argv=None) -> int:
     t_train = time.perf_counter() - t0
 
     # Held-out evaluation against a fresh playout mean.
-    ev_planes, ev_y = positions.value_labels(
-        args.eval_n, args.eval_seed0, args.eval_playouts,
-        args.eval_playout_seed0, threads=args.threads)
+    ev_planes, ev_y = make_labels(args.eval_n, args.eval_seed0,
+                                  args.eval_playouts, args.eval_playout_seed0)
     pred = sur.predict(ev_planes)
     out = {"mae_surrogate": float(np.mean(np.abs(pred - ev_y))),
            "mae_constant": float(np.mean(np.abs(labels.mean() - ev_y)))}
     for k in (1, 4, 16, 64):
-        _, est = positions.value_labels(
-            args.eval_n, args.eval_seed0, k, args.eval_playout_seed0 + 7_000_000,
-            threads=args.threads, planes=False)
+        _, est = make_labels(args.eval_n, args.eval_seed0, k,
+                             args.eval_playout_seed0 + 7_000_000, planes=False)
         out[f"mae_{k}_playouts"] = float(np.mean(np.abs(est - ev_y)))
     # Fresh labels of training positions (generalisation gap).
     m = min(args.eval_n, args.n)
-    _, tr_fresh = positions.value_labels(
-        m, args.seed0, args.eval_playouts, args.eval_playout_seed0 + 9_000_000,
-        threads=args.threads, planes=False)
+    _, tr_fresh = make_labels(m, args.seed0, args.eval_playouts,
+                              args.eval_playout_seed0 + 9_000_000,
+                              planes=False)
     out["mae_surrogate_train_positions"] = float(
         np.mean(np.abs(sur.predict(planes[:m]) - tr_fresh)))
 
@@ -148,6 +150,8 @@ def main(argv=None) -> int:
     t_po = time.perf_counter() - t0
     out.update({
         "train_positions": args.n, "label_playouts": args.playouts,
+        "labels": "host playouts" if args.host_labels else "GPU playouts",
+        "data_seconds": round(t_data, 2),
         "steps": steps, "batch": args.batch, "lr": args.lr,
         "first_loss": losses[0][1] if losses else None,
         "final_loss": losses[-1][1] if losses else None,
