@@ -54,8 +54,10 @@ def cpu_baseline(pol, val, planes, vplanes, budget_s: float, threads: int):
     bias + ELU, FP32) on a bounded sample of the same positions, with
     `threads` workers x 1 BLAS thread (Network.cpp:236-250) and with one."""
     import oracle
+    from leela_amd import positions
     blas = oracle.load_blas()
     pnet, vnet = oracle.Net(pol), oracle.Net(val)
+    pos = [positions.Position.random(1 + i) for i in range(len(planes))]
 
     def rate(t, budget):
         n = min(len(planes), 2 * t)
@@ -67,15 +69,24 @@ def cpu_baseline(pol, val, planes, vplanes, budget_s: float, threads: int):
         t0 = time.perf_counter()
         oracle.forward_f32(pnet, planes[:m], oracle.HEAD_POLICY, threads=t)
         oracle.forward_f32(vnet, vplanes[:m], oracle.HEAD_VALUE, threads=t)
-        return m, m / (time.perf_counter() - t0)
+        t_nn = time.perf_counter() - t0
+        # The feature encoders on the same positions and threads (netbench
+        # times them with the nets: each thread encodes, then evaluates).
+        t0 = time.perf_counter()
+        positions.encode_batch(pos[:m], positions.NET_POLICY, 0, t)
+        positions.encode_batch(pos[:m], positions.NET_VALUE, 0, t)
+        t_enc = time.perf_counter() - t0
+        return m, m / t_nn, m / (t_nn + t_enc)
 
-    m, v = rate(threads, budget_s)
-    m1, v1 = rate(1, budget_s / 3)
+    m, v, ve = rate(threads, budget_s)
+    m1, v1, ve1 = rate(1, budget_s / 3)
     return {"value": round(v, 2), "unit": "positions/s", "cores": threads,
             "kind": "port",
             "sample": f"{m} of the benchmark's positions, policy+value, "
                       f"{threads} worker threads x 1 BLAS thread, {blas}",
             "value_1_thread": round(v1, 2), "sample_1_thread": m1,
+            "with_encoder": round(ve, 2),
+            "with_encoder_1_thread": round(ve1, 2),
             "cpu_model": _cpu_model()}
 
 
@@ -357,6 +368,8 @@ def main() -> int:
         kernel = (f"fused_net<{head}> policy (13 convs + softmax, {B} "
                   f"boards per launch)")
     achieved = kflops / (avg_ms * 1e-3) / 1e12
+    w_bytes = 1664609 * (4 if head == "fp32" else 2)
+    algo_bytes = B * (32 * 361 + 2 * 4 * 361) + w_bytes
     peak = PEAK_TFLOPS[head]
 
     cpu = None
@@ -438,6 +451,10 @@ def main() -> int:
                             "WRITE_SIZE x1.0, calibrated on a known-size "
                             "copy)",
             "avg_launch_ms": round(avg_ms, 4),
+            # Planes in + logits and probabilities out per board, weights
+            # once per launch (16-bit), over the launch time.
+            "algorithmic_bytes": algo_bytes,
+            "algorithmic_gb_s": round(algo_bytes / (avg_ms * 1e-3) / 1e9, 2),
         },
         "cpu_baseline": cpu,
     }
