@@ -31,7 +31,7 @@ AGREEMENT_BAR = 0.995
 # x1.0); the summary with the commands is
 # profiles/fused_net_policy_hbm_traffic_b1024.md. Keyed by form.
 TRAFFIC_B1024 = {
-    "fp16": {"bytes": 56_680_000, "fetch_kib": 26203.7, "write_kib": 2944.0},
+    "fp16": {"bytes": 58_724_000, "fetch_kib": 27201.4, "write_kib": 2944.0},
     "bf16": {"bytes": 52_970_000, "fetch_kib": 24390.1, "write_kib": 2944.0},
     "fp32": {"bytes": 579_100_000, "fetch_kib": 190213.0,
              "write_kib": 185100.0},
@@ -100,9 +100,13 @@ def main() -> int:
                     choices=["auto", "fp32", "bf16", "fp16"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--kernel-only", choices=["fp32", "bf16", "fp16"],
+    ap.add_argument("--kernel-only",
+                    choices=["fp32", "bf16", "fp16", "encoder", "playouts"],
                     help="profiling workload: only `steps` policy forwards "
-                         "of this form (plus --copy-mib), no JSON contract")
+                         "of this form, or `steps` device-encoder launches "
+                         "over 4096 boards (both nets), or one launch of "
+                         "2048 positions x 64 GPU playouts (plus "
+                         "--copy-mib), no JSON contract")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="with --kernel-only: a fill and an elementwise copy "
                          "of this many MiB, the known-size reference for PMC "
@@ -175,10 +179,33 @@ def main() -> int:
                 "logits": logits, "win": win}
 
     if args.kernel_only:
-        p = leela_amd.PRECISIONS[args.kernel_only]
-        for _ in range(args.warmup + args.steps):
-            be.forward_device(leela_amd.NET_POLICY, p, B, d_planes.data_ptr(),
-                              d_praw.data_ptr(), d_pout.data_ptr(), stream)
+        if args.kernel_only == "encoder":
+            n = 4 * B
+            enc = positions.DeviceEncoder(0, max_batch=n)
+            eb = positions.PositionBatch(
+                [positions.Position.random(1 + i) for i in range(n)])
+            epl = torch.empty((n, 32, 361), dtype=torch.uint8, device=dev)
+            evpl = torch.empty_like(epl)
+            for _ in range(args.warmup + args.steps):
+                enc.encode_both_into(eb, epl.data_ptr(), evpl.data_ptr(), 0,
+                                     stream)
+            torch.cuda.synchronize(dev)
+            enc.close()
+        elif args.kernel_only == "playouts":
+            n = 2048
+            enc = positions.DeviceEncoder(0, max_batch=n)
+            pb = positions.PositionBatch(
+                [positions.Position.random(10000 + i) for i in range(n)])
+            w = torch.empty(n, dtype=torch.float32, device=dev)
+            enc.playouts_into(pb, 64, 5, w.data_ptr(), stream)
+            torch.cuda.synchronize(dev)
+            enc.close()
+        else:
+            p = leela_amd.PRECISIONS[args.kernel_only]
+            for _ in range(args.warmup + args.steps):
+                be.forward_device(leela_amd.NET_POLICY, p, B,
+                                  d_planes.data_ptr(), d_praw.data_ptr(),
+                                  d_pout.data_ptr(), stream)
         if args.copy_mib:
             # Fill (writes N MiB), then y = x + 1 (reads N MiB, writes N
             # MiB): the known-size kernels that calibrate the byte counters.
