@@ -4,7 +4,9 @@
 //
 // Two record slots alternate, so packing the next batch overlaps the copy
 // and kernel of the previous one; a slot is reused once the event recorded
-// after its kernel has completed.
+// after its kernel has completed. Record copies go on the encoder's own
+// stream and the caller's stream waits for them, so a copy overlaps the
+// work still queued ahead of the kernel.
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -28,9 +30,13 @@ struct lz_encoder {
     struct Slot {
         lz::EncRecord *host = nullptr;  // pinned
         lz::EncRecord *dev = nullptr;
-        hipEvent_t done = nullptr;
+        hipEvent_t done = nullptr;    // after the kernel that reads it
+        hipEvent_t copied = nullptr;  // after its copy to the device
         bool used = false;
     } slot[2];
+    // The record copies run on their own stream, so they overlap whatever
+    // the caller's stream is still computing; the kernel waits for them.
+    hipStream_t copy_stream = nullptr;
     int next = 0;
     uint32_t *half_wins = nullptr;  // playout outcome counters, max_batch
 
@@ -65,10 +71,17 @@ int lz_encoder_create(lz_encoder **out, int device, int max_batch) {
                           hipHostMallocDefault) != hipSuccess ||
             hipMalloc(reinterpret_cast<void **>(&s.dev), bytes) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) !=
+                hipSuccess ||
+            hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) !=
                 hipSuccess) {
             lz_encoder_destroy(e);
             return LZ_ERR_NOMEM;
         }
+    }
+    if (hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking) !=
+        hipSuccess) {
+        lz_encoder_destroy(e);
+        return LZ_ERR_HIP;
     }
     if (hipMalloc(reinterpret_cast<void **>(&e->half_wins),
                   sizeof(uint32_t) * (size_t)max_batch) != hipSuccess) {
@@ -87,9 +100,11 @@ void lz_encoder_destroy(lz_encoder *e) {
             (void)hipEventSynchronize(s.done);
             (void)hipEventDestroy(s.done);
         }
+        if (s.copied) (void)hipEventDestroy(s.copied);
         if (s.dev) (void)hipFree(s.dev);
         if (s.host) (void)hipHostFree(s.host);
     }
+    if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
     if (e->half_wins) (void)hipFree(e->half_wins);
     delete e;
 }
@@ -122,9 +137,11 @@ int send_records(lz_encoder *e, const lz_position *const *pos, int n,
         const int S = pos[i]->st.board.side();
         off += (int64_t)lz::kPlanes * S * S;
     }
-    const hipError_t err = hipMemcpyAsync(sl.dev, sl.host,
-                                          sizeof(lz::EncRecord) * (size_t)n,
-                                          hipMemcpyHostToDevice, s);
+    hipError_t err = hipMemcpyAsync(sl.dev, sl.host,
+                                    sizeof(lz::EncRecord) * (size_t)n,
+                                    hipMemcpyHostToDevice, e->copy_stream);
+    if (err == hipSuccess) err = hipEventRecord(sl.copied, e->copy_stream);
+    if (err == hipSuccess) err = hipStreamWaitEvent(s, sl.copied, 0);
     if (err != hipSuccess)
         return e->fail(LZ_ERR_HIP, "record copy: %s", hipGetErrorString(err));
     *out = &sl;
