@@ -117,3 +117,18 @@ def test_bad_arguments_rejected(enc):
     with pytest.raises(LeelaError):
         enc.encode_into([Position.random(1)] * 1025, buf.data_ptr())
     enc.encode_into([], buf.data_ptr())  # empty batch: nothing to do
+
+
+def test_crafted_ladder_positions_match_host(enc):
+    """1,024 crafted ladder-rich positions (ladder starters and random
+    breakers, as in the golden fixture but on fresh seeds): the device
+    encoder equals the host encoder on both nets."""
+    from crafted import crafted_moves
+    pos = [replay(19, crafted_moves(20000 + s)) for s in range(1024)]
+    for net in (NET_POLICY, NET_VALUE):
+        got = enc.encode(pos, net).cpu().numpy()
+        want = encode_batch(pos, net)
+        assert np.array_equal(got, want), (net, np.argwhere(got != want)[:5])
+    # These positions do read ladders: both ladder planes are set somewhere.
+    pl = encode_batch(pos, NET_POLICY)
+    assert pl[:, 25].sum() > 20 and pl[:, 26].sum() > 100
