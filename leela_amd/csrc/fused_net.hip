@@ -179,26 +179,33 @@ __device__ __forceinline__ void store_tile(T *act,
     }
 }
 
-template <int kWaves>
-__device__ float block_sum(float v, float *s_red) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    float r = 0.0f;
-    for (int i = 0; i < kWaves; i++) r += s_red[i];
-    return r;
-}
-
-template <int kWaves>
-__device__ float block_max(float v, float *s_red) {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    float r = s_red[0];
-    for (int i = 1; i < kWaves; i++) r = fmaxf(r, s_red[i]);
-    return r;
+// A board's policy outputs from its head plane v (SS <= 384 pixels), by
+// one wavefront: raw = v and out = softmax(v / T) (Network.cpp:450-469).
+// Wave-level reductions only, so the workgroup's other waves go on to the
+// next slot meanwhile (its staging and convs do not touch v).
+__device__ void wave_softmax(const float *v, int SS, float temperature,
+                             float *raw, float *out, int lane) {
+    float mx = -INFINITY;
+    for (int p = lane; p < SS; p += 64) mx = fmaxf(mx, v[p]);
+    for (int off = 32; off > 0; off >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    const float alpha = mx / temperature;
+    float e[6], sum = 0.f;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int p = lane + 64 * i;
+        e[i] = p < SS ? expf(v[p] / temperature - alpha) : 0.f;
+        sum += e[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int p = lane + 64 * i;
+        if (p < SS) {
+            if (raw) raw[p] = v[p];
+            if (out) out[p] = e[i] / sum;
+        }
+    }
 }
 
 // ---- FP8 e4m3 trunk ---------------------------------------------------------
@@ -430,27 +437,19 @@ __global__ __launch_bounds__(kWM * kWN * 64, 2) void fused_net_f8(
             }
         }
         __syncthreads();
-        if (VALUE) {
-            float *h = args.h0 + (size_t)board * kP;
-            for (int p = tid; p < kP; p += kThreads) h[p] = s_vec[p];
-        } else {
-            const size_t o = (size_t)board * kP;
-            float mx = -INFINITY;
-            for (int p = tid; p < SS; p += kThreads) mx = fmaxf(mx, s_vec[p]);
-            mx = block_max<kWaves>(mx, s_red);
-            const float alpha = mx / temperature;
-            float e[2] = {0.f, 0.f}, sum = 0.f;
-            for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
-                e[i] = expf(s_vec[p] / temperature - alpha);
-                sum += e[i];
-            }
-            sum = block_sum<kWaves>(sum, s_red);
-            for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
-                if (raw) raw[o + p] = s_vec[p];
-                if (out) out[o + p] = e[i] / sum;
+        // Outputs by wave 0 alone (no barrier): the next slot's staging and
+        // convs do not touch s_vec, and its head conv comes after several
+        // barriers that wave 0 reaches only once done here.
+        if (wave == 0) {
+            if (VALUE) {
+                float *h = args.h0 + (size_t)board * kP;
+                for (int p = lane; p < kP; p += 64) h[p] = s_vec[p];
+            } else {
+                const size_t o = (size_t)board * kP;
+                wave_softmax(s_vec, SS, temperature, raw ? raw + o : nullptr,
+                             out ? out + o : nullptr, lane);
             }
         }
-        __syncthreads();
     }
 }
 
@@ -483,7 +482,6 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
 
     __shared__ __attribute__((aligned(16))) T s_act[TILE];
     __shared__ float s_vec[384];
-    __shared__ float s_red[kWaves];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / kWN, wn = wave % kWN;
@@ -581,34 +579,26 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
             }
         }
         __syncthreads();
-        for (int kb = 0; kb < k; kb++) {
-            const float *v = s_vec + kb * SS;
-            if (VALUE) {
-                float *h = args.h0 + (size_t)board[kb] * kP;
-                for (int p = tid; p < kP; p += kThreads) {
-                    const int y = p / 19, x = p % 19;
-                    h[p] = (y < S && x < S) ? v[y * S + x] : 0.0f;
-                }
-            } else {
-                const size_t o = RAGGED ? (size_t)args.out_off[board[kb]]
-                                        : (size_t)board[kb] * kP;
-                float mx = -INFINITY;
-                for (int p = tid; p < SS; p += kThreads) mx = fmaxf(mx, v[p]);
-                mx = block_max<kWaves>(mx, s_red);
-                const float alpha = mx / temperature;
-                float e[2] = {0.f, 0.f}, sum = 0.f;
-                for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
-                    e[i] = expf(v[p] / temperature - alpha);
-                    sum += e[i];
-                }
-                sum = block_sum<kWaves>(sum, s_red);
-                for (int i = 0, p = tid; p < SS; p += kThreads, i++) {
-                    if (raw) raw[o + p] = v[p];
-                    if (out) out[o + p] = e[i] / sum;
+        // Outputs by wave 0 alone (no barrier): the next slot's staging and
+        // convs do not touch s_vec, and its head conv comes after several
+        // barriers that wave 0 reaches only once done here.
+        if (wave == 0) {
+            for (int kb = 0; kb < k; kb++) {
+                const float *v = s_vec + kb * SS;
+                if (VALUE) {
+                    float *h = args.h0 + (size_t)board[kb] * kP;
+                    for (int p = lane; p < kP; p += 64) {
+                        const int y = p / 19, x = p % 19;
+                        h[p] = (y < S && x < S) ? v[y * S + x] : 0.0f;
+                    }
+                } else {
+                    const size_t o = RAGGED ? (size_t)args.out_off[board[kb]]
+                                            : (size_t)board[kb] * kP;
+                    wave_softmax(v, SS, temperature, raw ? raw + o : nullptr,
+                                 out ? out + o : nullptr, lane);
                 }
             }
         }
-        __syncthreads();
     }
 }
 
