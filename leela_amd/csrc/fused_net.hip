@@ -62,7 +62,48 @@ constexpr int kWN = 4;                // waves along pixels
 constexpr int kNT = 6;                // 16-pixel tiles per wave
 constexpr int kP = 361;
 constexpr int kInPS = 40;             // pixel stride of the 32-plane input
-constexpr int kTilePixels = 456;      // LDS tile capacity in pixels
+constexpr int kTilePixels = 456;      // slot capacity in tile pixels
+
+// Boards of `side` one ragged slot holds: as many as fit the 384 pixels of
+// the column tiling and kTilePixels tile pixels (1 for 19x19, 2 for 13x13,
+// 4 for 9x9), at most 4.
+constexpr int slot_capacity(int side) {
+    if (side < 5 || side > 19) return 0;
+    int k = 1;
+    while (k < 4 && (k + 1) * side * side <= 384 &&
+           (1 + (k + 1) * (side + 1)) * (side + 2) <= kTilePixels)
+        k++;
+    return k;
+}
+
+// Padding (in elements) at the end of each LDS tile row. A 16-pixel
+// column tile runs along a board row and wraps onto the next one, skipping
+// the `halo` zero columns between them. Its B-fragment reads are one
+// 16-byte unit per lane; the pixel pitch (ps / 8 units) is odd, so
+// consecutive pixels of a row fall on distinct 16-byte bank groups (16 of
+// them across the 64 4-byte banks), and the pad makes the row pitch, in
+// units, congruent mod 16 to the S pixels of the row before it: the pixels
+// after a wrap continue the bank sequence as if no halo were between them.
+// Without it, fragments straddling a row end hit the same banks twice.
+constexpr int row_pad(int ps, int halo) {
+    return 8 * ((((-halo * (ps / 8)) % 16) + 16) % 16);
+}
+
+// LDS elements of a slot's tile: the stem's input (pitch S + 4, 32
+// channels at kInPS) or the trunk (pitch S + 2, ps), whichever is larger,
+// over the sides a slot can hold.
+constexpr int tile_elems(int ps, bool ragged) {
+    int best = 0;
+    for (int S = ragged ? 5 : 19; S <= 19; S++) {
+        const int k = ragged ? slot_capacity(S) : 1;
+        const int trunk = (1 + k * (S + 1)) * ((S + 2) * ps + row_pad(ps, 2));
+        const int stem =
+            (2 + k * (S + 2)) * ((S + 4) * kInPS + row_pad(kInPS, 4));
+        best = trunk > best ? trunk : best;
+        best = stem > best ? stem : best;
+    }
+    return best;
+}
 
 __device__ __forceinline__ float elu(float v) {
     return v > 0.0f ? v : (expf(v) - 1.0f);
@@ -78,15 +119,15 @@ __device__ __forceinline__ float elu_operand(float v) {
                     : __builtin_amdgcn_exp2f(v * 1.44269504088896341f) - 1.0f;
 }
 
-// One same-padded conv of the tile `act` (row pitch sp, stride PS, CIN
-// input channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the
+// One same-padded conv of the tile `act` (row pitch rp elements, pixel
+// stride PS, CIN input channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the
 // packed weights) for the wave's column tiles; pb[t] is the lane's window
 // origin in column tile t.
 template <typename T, int F, int CIN, int MT, int MTW, int PS>
 __device__ __forceinline__ void conv_tile(const T *act,
                                           const typename Op<T>::v8 *__restrict__ wpk,
                                           f32x4 (&acc)[MTW][kNT],
-                                          const int (&pb)[kNT], int sp,
+                                          const int (&pb)[kNT], int rp,
                                           int mt0, int lane,
                                           const float *__restrict__ bias =
                                               nullptr) {
@@ -119,7 +160,7 @@ __device__ __forceinline__ void conv_tile(const T *act,
     for (int s = 0; s < STEPS; s++) {
         const int nxt = s + 1 < STEPS ? s + 1 : s;
         const int tap = nxt / CB, cb = nxt % CB;
-        const int off = ((tap / F) * sp + (tap % F)) * PS + cb * 32 + g8;
+        const int off = (tap / F) * rp + (tap % F) * PS + cb * 32 + g8;
         v8 bn[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++)
@@ -391,8 +432,8 @@ __global__ __launch_bounds__(kWM * kWN * 64, 2) void fused_net_f8(
         {
             f32x4 acc[MTW1][kNT];
             conv_tile<T, 5, 32, MT1, MTW1, kInPS>(
-                s_stem, static_cast<const v8 *>(args.w[0]), acc, pb_in, pin,
-                wm * MTW1, lane);
+                s_stem, static_cast<const v8 *>(args.w[0]), acc, pb_in,
+                pin * kInPS, wm * MTW1, lane);
             __syncthreads();
             for (int i = tid * 16; i < TRUNK; i += kThreads * 16)
                 *reinterpret_cast<v4i *>(s_t8 + i) = v4i{0, 0, 0, 0};
@@ -471,14 +512,14 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
     float temperature) {
     typedef typename Op<T>::v8 v8;
     constexpr int PS = C + 8;
-    constexpr int TILE = (RAGGED ? kTilePixels : 21 * 21) * PS;
+    constexpr int RPAD = row_pad(PS, 2), RPAD_IN = row_pad(kInPS, 4);
+    constexpr int TILE = tile_elems(PS, RAGGED);
     constexpr int kWaves = kWM * kWN;
     constexpr int kThreads = kWaves * 64;
     constexpr int MT = C / 16, MTW = MT / kWM;
     constexpr int MT1 = M1 / 16, MTW1 = MT1 / kWM;
     static_assert(C % 32 == 0 && M1 % 32 == 0 && MT % kWM == 0 &&
                       MT1 % kWM == 0, "channel blocking");
-    static_assert(23 * 23 * kInPS <= TILE, "stem tile fits the trunk tile");
 
     __shared__ __attribute__((aligned(16))) T s_act[TILE];
     __shared__ float s_vec[384];
@@ -500,7 +541,8 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
             for (int i = 0; i < 4; i++) board[i] = sl.board[i];
         }
         const int SS = S * S, npix = k * SS;
-        const int pin = S + 4, pt = S + 2;  // stem / trunk row pitch
+        const int pin = S + 4, pt = S + 2;  // stem / trunk row pixels
+        const int rp_in = pin * kInPS + RPAD_IN, rp = pt * PS + RPAD;
         int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++) {
@@ -509,9 +551,9 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
             pix[t] = j;
             const int jj = j >= 0 ? j : 0;
             const int kb = jj / SS, r = jj % SS, y = r / S, x = r % S;
-            pb_in[t] = ((kb * (S + 2) + y) * pin + x) * kInPS;
-            pb[t] = ((kb * (S + 1) + y) * pt + x) * PS;
-            po[t] = j >= 0 ? pb[t] + (pt + 1) * PS : -1;
+            pb_in[t] = (kb * (S + 2) + y) * rp_in + x * kInPS;
+            pb[t] = (kb * (S + 1) + y) * rp + x * PS;
+            po[t] = j >= 0 ? pb[t] + rp + PS : -1;
         }
 
         // Stage the 32 binary planes with a 2-wide zero halo.
@@ -529,7 +571,8 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
 #pragma unroll
             for (int e = 0; e < 8; e++)
                 v[e] = (T)(in ? (float)src[(cg + e) * SS + yy * S + xx] : 0.0f);
-            *reinterpret_cast<v8 *>(s_act + pp * kInPS + cg) = v;
+            *reinterpret_cast<v8 *>(s_act + (row + 2) * rp_in +
+                                    (xx + 2) * kInPS + cg) = v;
         }
         __syncthreads();
 
@@ -537,11 +580,11 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         {
             f32x4 acc[MTW1][kNT];
             conv_tile<T, 5, 32, MT1, MTW1, kInPS>(s_act, W(0), acc, pb_in,
-                                                  pin, wm * MTW1, lane,
+                                                  rp_in, wm * MTW1, lane,
                                                   args.b[0]);
             __syncthreads();
             // Switch to the trunk geometry: clear tile and halos, then write.
-            const int trunk = (1 + k * (S + 1)) * pt * PS;
+            const int trunk = (1 + k * (S + 1)) * rp;
             for (int i = tid * 8; i < trunk; i += kThreads * 8)
                 *reinterpret_cast<v8 *>(s_act + i) = v8{};
             __syncthreads();
@@ -551,7 +594,7 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         // Middle 3x3 convs; the first reads M1 channels.
         {
             f32x4 acc[MTW][kNT];
-            conv_tile<T, 3, M1, MT, MTW, PS>(s_act, W(1), acc, pb, pt,
+            conv_tile<T, 3, M1, MT, MTW, PS>(s_act, W(1), acc, pb, rp,
                                              wm * MTW, lane, args.b[1]);
             __syncthreads();
             store_tile<T, MTW>(s_act, acc, wm * MTW, po, lane);
@@ -559,7 +602,7 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         }
         for (int l = 2; l <= NMID; l++) {
             f32x4 acc[MTW][kNT];
-            conv_tile<T, 3, C, MT, MTW, PS>(s_act, W(l), acc, pb, pt,
+            conv_tile<T, 3, C, MT, MTW, PS>(s_act, W(l), acc, pb, rp,
                                             wm * MTW, lane, args.b[l]);
             __syncthreads();
             store_tile<T, MTW>(s_act, acc, wm * MTW, po, lane);
@@ -569,7 +612,7 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         // pixels.
         if (wm == 0) {
             f32x4 acc[1][kNT];
-            conv_tile<T, 3, C, 1, 1, PS>(s_act, W(NMID + 1), acc, pb, pt, 0,
+            conv_tile<T, 3, C, 1, 1, PS>(s_act, W(NMID + 1), acc, pb, rp, 0,
                                          lane);
             const float hb = args.b[NMID + 1][0];
             if (lane < 16) {
@@ -704,14 +747,7 @@ int fused_blocks_per_cu(int which, int wm) {
     return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
-int ragged_slot_capacity(int side) {
-    if (side < 5 || side > 19) return 0;
-    int k = 1;
-    while (k < 4 && (k + 1) * side * side <= 384 &&
-           (1 + (k + 1) * (side + 1)) * (side + 2) <= kTilePixels)
-        k++;
-    return k;
-}
+int ragged_slot_capacity(int side) { return slot_capacity(side); }
 
 hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
                          const FusedArgs &args, float *raw, float *out, int n,
