@@ -9,8 +9,9 @@
 //     side (two 13x13, four 9x9: ragged batches). The slot's activations
 //     live in LDS as 16-bit values in an HWC tile with zero halos:
 //     boards are stacked vertically with one zero separator row between
-//     them, row pitch S + 2, pixel stride PS = C + 8 (the 8-element pad
-//     staggers pixels across LDS banks).
+//     them, row pitch S + 2 pixels plus a row pad, pixel stride PS = C + 16
+//     elements (the LDS bank layout that these choose is given with
+//     row_pad below).
 //   * Each conv is an implicit GEMM D[cout x pixels] = W[cout x K] B[K x
 //     pixels], K = k*k*cin ordered tap-major, 32 channels per MFMA k-step
 //     (v_mfma_f32_16x16x32_{bf16,f16}). The waves form a WM (output
@@ -61,7 +62,7 @@ struct Op<_Float16> {
 constexpr int kWN = 4;                // waves along pixels
 constexpr int kNT = 6;                // 16-pixel tiles per wave
 constexpr int kP = 361;
-constexpr int kInPS = 40;             // pixel stride of the 32-plane input
+constexpr int kInPS = 48;             // pixel stride of the 32-plane input
 constexpr int kTilePixels = 456;      // slot capacity in tile pixels
 
 // Boards of `side` one ragged slot holds: as many as fit the 384 pixels of
@@ -76,17 +77,22 @@ constexpr int slot_capacity(int side) {
     return k;
 }
 
-// Padding (in elements) at the end of each LDS tile row. A 16-pixel
-// column tile runs along a board row and wraps onto the next one, skipping
-// the `halo` zero columns between them. Its B-fragment reads are one
-// 16-byte unit per lane; the pixel pitch (ps / 8 units) is odd, so
-// consecutive pixels of a row fall on distinct 16-byte bank groups (16 of
-// them across the 64 4-byte banks), and the pad makes the row pitch, in
-// units, congruent mod 16 to the S pixels of the row before it: the pixels
-// after a wrap continue the bank sequence as if no halo were between them.
-// Without it, fragments straddling a row end hit the same banks twice.
+// LDS bank layout of the tiles. A 16-byte ds_read (the B fragments of the
+// convs) is served per half-wavefront, from 128 4-byte banks: the 32 lanes
+// of a half-wave are conflict-free when their 16-byte units are distinct mod
+// 32 (fitted to rocprofv3's SQ_LDS_BANK_CONFLICT over a set of lane
+// patterns on gfx950). Lanes 0-31 of a fragment read 16 pixels x two
+// 16-byte channel groups (units u*pixel + 0 and + 1, u = PS / 8). With u = 2
+// mod 4 (PS = C + 16: 18 units for 128 channels, 10 for 64; 6 for the
+// 32-plane input at 48) 16 consecutive pixels take the 16 even residues and
+// the second channel group the odd ones. A 16-pixel column tile also wraps
+// from the end of a board row to the next row, skipping the `halo` zero
+// columns; the pad at the end of each row makes the row pitch, in units,
+// congruent mod 32 to u times the S pixels of the row, so the pixels after
+// a wrap continue the same residue sequence. row_pad is that pad, in
+// elements (u * halo + pad = 0 mod 32 units).
 constexpr int row_pad(int ps, int halo) {
-    return 8 * ((((-halo * (ps / 8)) % 16) + 16) % 16);
+    return 8 * ((((-halo * (ps / 8)) % 32) + 32) % 32);
 }
 
 // LDS elements of a slot's tile: the stem's input (pitch S + 4, 32
@@ -150,48 +156,65 @@ __device__ __forceinline__ void conv_tile(const T *act,
     }
 
     const v8 *wl = wpk + mt0 * 64 + lane;
-    v8 a[MTW], b[kNT];
+    // Operands of k-step s, clamped to the last step (the loads issued past
+    // the end re-read it and go unused).
+    auto b_off = [&](int s) {
+        s = s < STEPS ? s : STEPS - 1;
+        const int tap = s / CB, cb = s % CB;
+        return (tap / F) * rp + (tap % F) * PS + cb * 32 + g8;
+    };
+    auto a_at = [&](int s, int m) {
+        s = s < STEPS ? s : STEPS - 1;
+        return wl[(s * MT + m) * 64];
+    };
+    // B fragments (LDS) are loaded one k-step ahead into the other of two
+    // register sets; the step loop is unrolled by two so that the sets
+    // alternate without copies.
+    v8 a[MTW], b0[kNT], b1[kNT];
 #pragma unroll
-    for (int m = 0; m < MTW; m++) a[m] = wl[m * 64];
+    for (int m = 0; m < MTW; m++) a[m] = a_at(0, m);
 #pragma unroll
     for (int t = 0; t < kNT; t++)
-        b[t] = *reinterpret_cast<const v8 *>(act + pb[t] + g8);
+        b0[t] = *reinterpret_cast<const v8 *>(act + pb[t] + g8);
 
-    for (int s = 0; s < STEPS; s++) {
-        const int nxt = s + 1 < STEPS ? s + 1 : s;
-        const int tap = nxt / CB, cb = nxt % CB;
-        const int off = (tap / F) * rp + (tap % F) * PS + cb * 32 + g8;
-        v8 bn[kNT];
+    // Step s: acc += a * b while the B fragments of step s + 1 load into
+    // bn. The MFMAs run m-tile by m-tile at raised wave priority; each A
+    // fragment reloads with step s + 1's right after its last MFMA (a full
+    // step of MFMAs ahead of its next use: the loads come from L2), and
+    // the kNT LDS reads are spread over the first half of the MFMAs.
+    auto step = [&](const v8(&b)[kNT], v8(&bn)[kNT], int s) {
+        const int off = b_off(s + 1);
 #pragma unroll
         for (int t = 0; t < kNT; t++)
             bn[t] = *reinterpret_cast<const v8 *>(act + pb[t] + off);
-        // The MFMA burst runs at raised wave priority, and the next step's
-        // operand loads are spread through it: the first half of the
-        // MFMAs interleaved with the kNT LDS reads, the second half with
-        // the MTW global loads (measured 4% faster than the compiler's own
-        // order, which issues all LDS reads together right before use).
         __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int m = 0; m < MTW; m++) {
 #pragma unroll
             for (int t = 0; t < kNT; t++)
                 acc[m][t] = Op<T>::mfma(a[m], b[t], acc[m][t]);
-            a[m] = wl[(nxt * MT + m) * 64];
+            a[m] = a_at(s + 1, m);
         }
-        constexpr int kHalf = MTW * kNT / 2;
+        constexpr int kMfma = MTW * kNT;
+        constexpr int kDsEvery = (kMfma / 2) / kNT > 0 ? (kMfma / 2) / kNT : 1;
+        int ds = 0;
 #pragma unroll
-        for (int i = 0; i < kNT; i++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, kHalf / kNT, 0);  // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);            // DS read
-        }
+        for (int m = 0; m < MTW; m++) {
 #pragma unroll
-        for (int i = 0; i < MTW; i++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, kHalf / MTW, 0);  // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);            // VMEM read
+            for (int t = 0; t < kNT; t++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                if ((m * kNT + t) % kDsEvery == kDsEvery - 1 && ds < kNT) {
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+                    ds++;
+                }
+            }
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
         }
         __builtin_amdgcn_s_setprio(0);
-#pragma unroll
-        for (int t = 0; t < kNT; t++) b[t] = bn[t];
+    };
+    for (int s = 0; s < STEPS; s += 2) {
+        step(b0, b1, s);
+        if (s + 1 < STEPS) step(b1, b0, s + 1);
     }
 }
 
@@ -376,7 +399,8 @@ __device__ __forceinline__ float store_tile_f8(uint8_t *act,
 // from the layer's output range. 19x19 boards only (no ragged slots).
 // Outputs as fused_net.
 template <int C, int M1, int NMID, bool VALUE, int kWM>
-__global__ __launch_bounds__(kWM * kWN * 64, 2) void fused_net_f8(
+__global__ __launch_bounds__(kWM * kWN * 64,
+                             !VALUE && kWM == 1 ? 1 : 2) void fused_net_f8(
     const uint8_t *__restrict__ planes, FusedArgs args,
     float *__restrict__ raw, float *__restrict__ out, int nslots,
     float temperature) {
@@ -511,7 +535,9 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
     float *__restrict__ raw, float *__restrict__ out, int nslots,
     float temperature) {
     typedef typename Op<T>::v8 v8;
-    constexpr int PS = C + 8;
+    constexpr int PS = C + 16;
+    static_assert((PS / 8) % 4 == 2 && (kInPS / 8) % 4 == 2,
+                  "pixel strides of 2 mod 4 16-byte units");
     constexpr int RPAD = row_pad(PS, 2), RPAD_IN = row_pad(kInPS, 4);
     constexpr int TILE = tile_elems(PS, RAGGED);
     constexpr int kWaves = kWM * kWN;
@@ -546,14 +572,15 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
         int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
 #pragma unroll
         for (int t = 0; t < kNT; t++) {
-            int j = (nt0 + t) * 16 + (lane & 15);
-            if (j >= npix) j = -1;
-            pix[t] = j;
-            const int jj = j >= 0 ? j : 0;
+            const int j = (nt0 + t) * 16 + (lane & 15);
+            pix[t] = j < npix ? j : -1;
+            // A lane past the last pixel reads the pixel a multiple of 16
+            // before it (npix >= 25): the tile keeps its bank pattern.
+            const int jj = j < npix ? j : j - ((j - npix) / 16 + 1) * 16;
             const int kb = jj / SS, r = jj % SS, y = r / S, x = r % S;
             pb_in[t] = (kb * (S + 2) + y) * rp_in + x * kInPS;
             pb[t] = (kb * (S + 1) + y) * rp + x * PS;
-            po[t] = j >= 0 ? pb[t] + rp + PS : -1;
+            po[t] = j < npix ? pb[t] + rp + PS : -1;
         }
 
         // Stage the 32 binary planes with a 2-wide zero halo.
