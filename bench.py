@@ -31,7 +31,7 @@ AGREEMENT_BAR = 0.995
 # x1.0); the summary with the commands is
 # profiles/fused_net_policy_hbm_traffic_b1024.md. Keyed by form.
 TRAFFIC_B1024 = {
-    "fp16": {"bytes": 58_724_000, "fetch_kib": 27201.4, "write_kib": 2944.0},
+    "fp16": {"bytes": 82_946_000, "fetch_kib": 39029.2, "write_kib": 2944.0},
     "bf16": {"bytes": 52_970_000, "fetch_kib": 24390.1, "write_kib": 2944.0},
     "fp32": {"bytes": 579_100_000, "fetch_kib": 190213.0,
              "write_kib": 185100.0},

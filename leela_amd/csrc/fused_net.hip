@@ -155,7 +155,13 @@ __device__ __forceinline__ void conv_tile(const T *act,
         for (int t = 0; t < kNT; t++) acc[m][t] = init;
     }
 
-    const v8 *wl = wpk + mt0 * 64 + lane;
+    // (The empty asm keeps the compiler from hoisting the per-step
+    // fragment addresses of a fully unrolled conv, the 1-channel head's,
+    // out of the slot loop, where they would stay live, and spill, for the
+    // whole kernel.)
+    int lanev = lane;
+    asm volatile("" : "+v"(lanev));
+    const v8 *wl = wpk + mt0 * 64 + lanev;
     // Operands of k-step s, clamped to the last step (the loads issued past
     // the end re-read it and go unused).
     auto b_off = [&](int s) {
@@ -672,9 +678,9 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
     }
 }
 
-// Policy: one 8-wave workgroup per CU (121-124 KB LDS tile), 2 waves per
-// SIMD. Value: 4-wave workgroups, two per CU (65 KB tile each), 2 waves per
-// SIMD.
+// Policy: one 8-wave workgroup per CU (136 KB LDS tile, 148 KB in the
+// ragged build), 2 waves per SIMD. Value: 4-wave workgroups, two per CU
+// (75 / 80 KB tile each), 2 waves per SIMD.
 // Wave grids the autotuner chooses from. Policy: 2 x 4 (8 waves, 96
 // accumulators per lane, 2 waves/SIMD; default) or 1 x 4 (4 waves, 192
 // accumulators, 1 wave/SIMD). Value: 1 x 4 (4 waves, two workgroups per CU;
