@@ -379,6 +379,22 @@ def main() -> int:
     finally:
         be.queue_stop()
 
+    # Search-side batching (lz_search): search threads grow one tree from a
+    # generator position, each new leaf expanded by the policy net and
+    # valued by the value net through the synchronous facade calls, which
+    # the leaf queue batches across threads; one thread without the queue
+    # for comparison.
+    from leela_amd import search as lz_search
+    s_root = positions.Position.random(21)
+    s_threads, s_sims = 128, 4096
+    be.queue_start(p, max_batch=256, max_wait_us=300)
+    try:
+        srch = lz_search.search(s_root, s_threads, s_sims, backend=be,
+                                precision=p, seed=4)
+    finally:
+        be.queue_stop()
+    srch1 = lz_search.search(s_root, 1, 256, backend=be, precision=p, seed=4)
+
     # Dominant kernel, timed per launch with HIP events on its stream.
     kern_ms = []
     settle(p)
@@ -456,6 +472,18 @@ def main() -> int:
                         "policy then 10000 value evaluations, "
                         "RANDOM_ROTATION, synchronous calls from 16 "
                         "threads through the leaf queue (max batch 16)",
+            },
+            "search": {
+                "threads": s_threads,
+                "simulations": int(srch["simulations"]),
+                "simulations_per_s": round(srch["simulations_per_s"], 1),
+                "mean_batch": round(srch["mean_batch"], 1),
+                "collisions": int(srch["collisions"]),
+                "simulations_per_s_1_thread": round(
+                    srch1["simulations_per_s"], 1),
+                "what": "lz_search: tree search with virtual loss, each new "
+                        "leaf one policy and one value row through the leaf "
+                        "queue (max batch 256); 1 thread: direct forwards",
             },
             "playouts": {
                 "device_playouts_per_s": round(gpu_po_s, 1),

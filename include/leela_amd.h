@@ -323,6 +323,46 @@ typedef void (*lz_moves_fn)(void *user, int count, const float *probs,
 int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
                                   int rotation, lz_moves_fn done, void *user);
 
+/* ---- Search-side batching ------------------------------------------------
+ * UCTNode::netscore_children / UCTNode::run_value_net feeding the batched
+ * backend (UCTNode.cpp:74-121, 316-344; UCTSearch.cpp:79-149): `threads`
+ * search threads grow one tree from `root` for `simulations` simulations,
+ * with virtual loss on the path of each simulation in flight. With
+ * LZ_EVAL_NETS every new leaf is expanded with the policy net (priors over
+ * the Netresult's points that pass UCTNode::create_children's filter,
+ * UCTNode.cpp:149-158) and valued with the value net, both through the
+ * synchronous facade calls, which share launches through the leaf queue
+ * when one runs at `precision` (lz_queue_start); this replaces the
+ * reference's one blocking forward per search thread and its
+ * thread_can_issue backpressure (OpenCL.cpp:446-454). LZ_EVAL_PLAYOUTS
+ * (host only, ctx may be NULL) uses uniform priors and the mean of
+ * `playouts` light playouts (lz_playout_winrate) instead. Selection is
+ * Q + c_puct * P * sqrt(N) / (1 + n), not the reference's UCT-RAVE tree
+ * policy (UCTNode.cpp:543-700), which stays on the caller's side. */
+#define LZ_EVAL_NETS 0
+#define LZ_EVAL_PLAYOUTS 1
+typedef struct {
+    double seconds;
+    int64_t simulations;      /* completed simulations */
+    int64_t policy_evals;     /* leaf expansions (one policy row each) */
+    int64_t value_evals;      /* leaf evaluations (one value row each) */
+    int64_t collisions;       /* simulations retried: leaf being expanded */
+    int64_t nodes;            /* tree nodes */
+    int64_t queue_batches;    /* leaf-queue launches during the search */
+    int64_t queue_positions;  /* rows of those launches */
+    int64_t root_visits;
+    int64_t virtual_loss_left;    /* after the search; 0 when consistent */
+    int64_t visit_sum_mismatches; /* expanded nodes whose children's visits
+                                     do not sum to their own minus one */
+    int32_t best_move;        /* most visited root child (vertex, -1 pass) */
+    int32_t best_visits;
+    int32_t root_children;
+    float root_value;         /* mean backed-up value for the side to move */
+} lz_search_stats;
+int lz_search(lz_ctx *ctx, int precision, const lz_position *root,
+              int evaluator, int playouts, int threads, int simulations,
+              float c_puct, uint32_t seed, lz_search_stats *stats);
+
 /* ---- Monte Carlo playouts (teacher and fallback of the surrogate) -----
  * Playout::mc_owner (Playout.cpp:174-217): the mean outcome of n playouts
  * from pos for the side to move (win 1, tie 0.5). Each playout follows
