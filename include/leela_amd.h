@@ -330,7 +330,8 @@ int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
  * with virtual loss on the path of each simulation in flight. With
  * LZ_EVAL_NETS every new leaf is expanded with the policy net (priors over
  * the Netresult's points that pass UCTNode::create_children's filter,
- * UCTNode.cpp:149-158) and valued with the value net, both through the
+ * UCTNode.cpp:149-158, the 35 best as UCTNode.cpp:551-552 considers) and
+ * valued with the value net, both through the
  * synchronous facade calls, which share launches through the leaf queue
  * when one runs at `precision` (lz_queue_start); this replaces the
  * reference's one blocking forward per search thread and its
@@ -338,7 +339,9 @@ int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
  * (host only, ctx may be NULL) uses uniform priors and the mean of
  * `playouts` light playouts (lz_playout_winrate) instead. Selection is
  * Q + c_puct * P * sqrt(N) / (1 + n), not the reference's UCT-RAVE tree
- * policy (UCTNode.cpp:543-700), which stays on the caller's side. */
+ * policy (UCTNode.cpp:543-700), which stays on the caller's side.
+ * simulations <= 2^22; the tree holds up to 36 nodes per simulation
+ * (about 3 KB) with the nets, one per candidate point with playouts. */
 #define LZ_EVAL_NETS 0
 #define LZ_EVAL_PLAYOUTS 1
 typedef struct {

@@ -16,7 +16,7 @@
 // leaf's value with the sign alternating per ply. Children are the
 // Netresult's points that pass the filter of UCTNode::create_children
 // (UCTNode.cpp:149-158: not the ko square, not an own eye, not suicide),
-// or a pass when none does. A position after two passes in a row is
+// the 35 with the highest priors, or a pass when none does. A position after two passes in a row is
 // scored by FastBoard::final_mc_score.
 #include <algorithm>
 #include <atomic>
@@ -64,6 +64,8 @@ public:
     virtual bool expand(const go::GameState &st,
                         std::vector<std::pair<int, float>> &moves) = 0;
     virtual bool value(const go::GameState &st, uint32_t seed, float *v) = 0;
+    // Children kept per expansion, by prior (0: all).
+    virtual size_t max_children() const { return 0; }
 };
 
 bool candidate(const go::GameState &st, int v) {
@@ -92,6 +94,9 @@ public:
         *v = net.get_value(st, Network::DIRECT);
         return net.last_status() == LZ_OK;
     }
+    // The 35 best-scored children, as uct_select_child considers for a
+    // node with net scores (UCTNode.cpp:551-552).
+    size_t max_children() const override { return 35; }
 
 private:
     lz_ctx *ctx_;
@@ -181,6 +186,8 @@ struct Search {
                      const std::pair<int, float> &b) {
                       return a.second > b.second;
                   });
+        if (eval.max_children() && moves.size() > eval.max_children())
+            moves.resize(eval.max_children());
         leaf.children.reserve(moves.size());
         for (const auto &m : moves) {
             auto c = std::make_unique<Node>();
@@ -276,7 +283,7 @@ int lz_search(lz_ctx *ctx, int precision, const lz_position *root,
               int evaluator, int playouts, int threads, int simulations,
               float c_puct, uint32_t seed, lz_search_stats *stats) {
     if (!root || !stats || threads < 1 || threads > 1024 || simulations < 1 ||
-        !(c_puct > 0.0f))
+        simulations > (1 << 22) || !(c_puct > 0.0f))
         return LZ_ERR_ARG;
     if (evaluator != LZ_EVAL_NETS && evaluator != LZ_EVAL_PLAYOUTS)
         return LZ_ERR_ARG;
