@@ -149,22 +149,39 @@ __device__ Bits bits_fill(Bits s, const Bits &mask, int W) {
 
 // ---- the board in LDS ----------------------------------------------------
 // One board per workgroup (one wavefront): the record's squares and string
-// representatives, every string's stones by representative, liberty counts,
-// after-liberties, ladder flags and the ladder task list.
+// representatives, every string's stones by a dense string id, liberty
+// counts, after-liberties, ladder flags and the ladder task list.
+//
+// The first kStrCap strings (ids in representative order) keep their stones
+// in LDS; a board with more strings keeps the rest in its overflow area in
+// global memory (generator positions after 250 moves hold at most ~120
+// strings). The cap keeps the workgroup's LDS at 20 KB, eight workgroups
+// (two waves per SIMD) per CU.
+constexpr int kStrCap = kEncStrCap, kStrOvf = kEncStrOvf;
+static_assert(kEncStrWords == kWords, "string set words");
 
 __shared__ uint8_t s_sq[kEncGrid];
 __shared__ uint16_t s_head[kEncGrid];
 __shared__ uint16_t s_libs[kEncGrid];             // by representative
 __shared__ uint8_t s_aft[2][kEncGrid];            // after-liberties, capped 255
 __shared__ uint32_t s_lad[kEncGrid];              // bit 0 losing, bit 1 winning
-__shared__ uint64_t s_strings[kEncGrid][kWords];  // stones by representative
+__shared__ uint16_t s_sid[kEncGrid];              // string id by representative
+__shared__ uint64_t s_strings[kStrCap][kWords];   // stones by string id
 __shared__ uint32_t s_tasks[5 * 361];             // <= 4 winning + 1 losing per point
 __shared__ int s_ntasks;
+__shared__ uint64_t *s_ovf;  // this board's overflow strings (kStrOvf)
+
+// Word w of the stones of the string whose representative is h.
+__device__ __forceinline__ uint64_t str_word(int h, int w) {
+    const int id = s_sid[h];
+    return id < kStrCap ? s_strings[id][w]
+                        : s_ovf[(id - kStrCap) * kWords + w];
+}
 
 __device__ __forceinline__ Bits base_string(int h) {
     Bits s;
 #pragma unroll
-    for (int i = 0; i < kWords; i++) s.w[i] = s_strings[h][i];
+    for (int i = 0; i < kWords; i++) s.w[i] = str_word(h, i);
     return s;
 }
 
@@ -223,7 +240,7 @@ __device__ __forceinline__ int count_empty(int nc) {
 __device__ uint64_t string_at(const WBoard &b, const Grp &g, int col, int v) {
     uint64_t seed = w_point(g, v);
     if (s_sq[v] == col) {
-        const uint64_t base = g.gl < kWords ? s_strings[s_head[v]][g.gl] : 0ull;
+        const uint64_t base = g.gl < kWords ? str_word(s_head[v], g.gl) : 0ull;
         if (!w_any(base & ~b.c[col], g)) seed |= base;
     }
     if (col == b.hc && w_test(b.h | b.hn, g, v) && !w_any(b.h & ~b.c[col], g))
@@ -496,7 +513,8 @@ constexpr int kTaskLose = 1 << 20;
 __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
                                                     int symmetry,
                                                     uint8_t *out_policy,
-                                                    uint8_t *out_value) {
+                                                    uint8_t *out_value,
+                                                    uint64_t *ovf) {
     const int lane = threadIdx.x;
     const EncRecord &r = recs[blockIdx.x];
     const int side = r.side, W = side + 2, G = W * W, P = side * side;
@@ -511,7 +529,10 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
         uint32_t *e32 = reinterpret_cast<uint32_t *>(s_head);
         for (int i = lane; i < kEncGrid / 2; i += 64) e32[i] = h32[i];
     }
-    if (lane == 0) s_ntasks = 0;
+    if (lane == 0) {
+        s_ntasks = 0;
+        s_ovf = ovf + (size_t)blockIdx.x * kStrOvf * kWords;
+    }
     __syncthreads();
     const int d[4] = {-W, 1, W, -1};  // FastBoard::m_dirs order
     Bits stones[2], on;
@@ -525,30 +546,49 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
     }
     const Bits E = bits_andnot(on, bits_or(stones[0], stones[1]));
 
-    // 2. Stones of every string, then its liberty count.
-    for (int v = lane; v < G; v += 64) {
-        s_lad[v] = 0;
-        const int c = s_sq[v];
-        if ((c == kBlack || c == kWhite) && s_head[v] == v) {
+    // 2. String ids (representatives in vertex order), the stones of every
+    // string, then its liberty count.
+    {
+        int base = 0;
+        for (int v0 = 0; v0 < G; v0 += 64) {
+            const int v = v0 + lane;
+            const int c = v < G ? s_sq[v] : kOff;
+            const bool head = (c == kBlack || c == kWhite) && s_head[v] == v;
+            const uint64_t m = __ballot(head);
+            if (head) {
+                const int id = base + __popcll(m & ((1ull << lane) - 1ull));
+                s_sid[v] = (uint16_t)id;
 #pragma unroll
-            for (int i = 0; i < kWords; i++) s_strings[v][i] = 0;
+                for (int i = 0; i < kWords; i++) {
+                    if (id < kStrCap)
+                        s_strings[id][i] = 0;
+                    else
+                        s_ovf[(id - kStrCap) * kWords + i] = 0;
+                }
+            }
+            base += __popcll(m);
+        }
+        for (int v = lane; v < G; v += 64) s_lad[v] = 0;
+    }
+    __syncthreads();
+    for (int v = lane; v < G; v += 64) {
+        const int c = s_sq[v];
+        if (c == kBlack || c == kWhite) {
+            const int id = s_sid[s_head[v]];
+            unsigned long long *w =
+                id < kStrCap
+                    ? reinterpret_cast<unsigned long long *>(
+                          &s_strings[id][v >> 6])
+                    : reinterpret_cast<unsigned long long *>(
+                          &s_ovf[(id - kStrCap) * kWords + (v >> 6)]);
+            atomicOr(w, 1ull << (v & 63));
         }
     }
     __syncthreads();
     for (int v = lane; v < G; v += 64) {
         const int c = s_sq[v];
-        if (c == kBlack || c == kWhite)
-            atomicOr(reinterpret_cast<unsigned long long *>(
-                         &s_strings[s_head[v]][v >> 6]),
-                     1ull << (v & 63));
-    }
-    __syncthreads();
-    for (int v = lane; v < G; v += 64) {
-        const int c = s_sq[v];
         if ((c == kBlack || c == kWhite) && s_head[v] == v) {
-            Bits s;
-#pragma unroll
-            for (int i = 0; i < kWords; i++) s.w[i] = s_strings[v][i];
+            const Bits s = base_string(v);
             s_libs[v] = (uint16_t)bits_count(bits_and(bits_nbrs(s, W), E));
         }
     }
@@ -579,7 +619,7 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
                 if (nh[k] < 0) continue;
                 Bits s;
 #pragma unroll
-                for (int i = 0; i < kWords; i++) s.w[i] = s_strings[nh[k]][i];
+                for (int i = 0; i < kWords; i++) s.w[i] = str_word(nh[k], i);
                 if (ncol[k] == col)
                     m = bits_or(m, s);
                 else if (s_libs[nh[k]] == 1)
@@ -631,7 +671,7 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
                 if (nh[k] >= 0 && ncol[k] == tm && s_libs[nh[k]] > 1) {
                     Bits s;
 #pragma unroll
-                    for (int i = 0; i < kWords; i++) s.w[i] = s_strings[nh[k]][i];
+                    for (int i = 0; i < kWords; i++) s.w[i] = str_word(nh[k], i);
                     u = bits_or(u, bits_and(bits_nbrs(s, W), E));
                 }
             }
@@ -646,7 +686,7 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
             if (!captures) {
                 Bits s;
 #pragma unroll
-                for (int i = 0; i < kWords; i++) s.w[i] = s_strings[nh[k]][i];
+                for (int i = 0; i < kWords; i++) s.w[i] = str_word(nh[k], i);
                 Bits l = bits_and(bits_nbrs(s, W), E);
                 bits_clear(l, v);
                 const int e = bits_first(l);
@@ -742,11 +782,12 @@ __global__ __launch_bounds__(64) void encode_kernel(const EncRecord *recs,
 }  // namespace
 
 hipError_t launch_encode(const EncRecord *d_rec, int n, int symmetry,
-                         uint8_t *d_policy, uint8_t *d_value,
+                         uint8_t *d_policy, uint8_t *d_value, uint64_t *d_ovf,
                          hipStream_t stream) {
     if (n <= 0 || (!d_policy && !d_value)) return hipSuccess;
+    if (!d_ovf) return hipErrorInvalidValue;
     hipLaunchKernelGGL(encode_kernel, dim3(n), dim3(64), 0, stream, d_rec,
-                       symmetry, d_policy, d_value);
+                       symmetry, d_policy, d_value, d_ovf);
     return hipGetLastError();
 }
 

@@ -11,6 +11,13 @@ namespace lz {
 // The grid is the padded (side+2)^2 board of the reference (FastBoard
 // vertex numbering), row-major.
 constexpr int kEncGrid = 448;  // >= 21 * 21, a multiple of 64
+// The device encoder keeps the stones of a board's first kEncStrCap strings
+// in LDS and the rest (a board has at most 361 strings) in an overflow area
+// of kEncStrOvf sets of kEncStrWords 64-bit words per board in global
+// memory, which the caller provides.
+constexpr int kEncStrCap = 128;
+constexpr int kEncStrOvf = 361 - kEncStrCap;
+constexpr int kEncStrWords = 7;  // kEncGrid bits
 
 struct alignas(64) EncRecord {
     uint8_t sq[kEncGrid];      // 0 black, 1 white, 2 empty, 3 off-board

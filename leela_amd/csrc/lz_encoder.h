@@ -14,8 +14,11 @@ namespace lz {
 // Planes of n boards (uint8 [32][S*S] each at out + out_offset) for the
 // policy net's layout into d_policy and the value net's into d_value
 // (either may be NULL), rotated by `symmetry` (0..7, Network.cpp:765-773).
+// d_ovf: n * kEncStrOvf * kEncStrWords words of device memory, the
+// overflow strings of boards with more strings than the kernel keeps in
+// LDS (lz_enc_record.h).
 hipError_t launch_encode(const EncRecord *d_rec, int n, int symmetry,
-                         uint8_t *d_policy, uint8_t *d_value,
+                         uint8_t *d_policy, uint8_t *d_value, uint64_t *d_ovf,
                          hipStream_t stream);
 
 // Light playouts (playout.cpp's policy) of n boards, nplay each, playout j

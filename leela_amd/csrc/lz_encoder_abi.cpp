@@ -32,6 +32,9 @@ struct lz_encoder {
         lz::EncRecord *dev = nullptr;
         hipEvent_t done = nullptr;    // after the kernel that reads it
         hipEvent_t copied = nullptr;  // after its copy to the device
+        // Overflow strings of the boards encoded from this slot (per slot:
+        // launches from both slots may run at once on different streams).
+        uint64_t *ovf = nullptr;
         bool used = false;
     } slot[2];
     // The record copies run on their own stream, so they overlap whatever
@@ -70,6 +73,9 @@ int lz_encoder_create(lz_encoder **out, int device, int max_batch) {
         if (hipHostMalloc(reinterpret_cast<void **>(&s.host), bytes,
                           hipHostMallocDefault) != hipSuccess ||
             hipMalloc(reinterpret_cast<void **>(&s.dev), bytes) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&s.ovf),
+                      sizeof(uint64_t) * lz::kEncStrOvf * lz::kEncStrWords *
+                          (size_t)max_batch) != hipSuccess ||
             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) !=
                 hipSuccess ||
             hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) !=
@@ -102,6 +108,7 @@ void lz_encoder_destroy(lz_encoder *e) {
         }
         if (s.copied) (void)hipEventDestroy(s.copied);
         if (s.dev) (void)hipFree(s.dev);
+        if (s.ovf) (void)hipFree(s.ovf);
         if (s.host) (void)hipHostFree(s.host);
     }
     if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
@@ -179,7 +186,7 @@ int encode(lz_encoder *e, const lz_position *const *pos, int n, int symmetry,
     rc = send_records(e, pos, n, s, &sl);
     if (rc != LZ_OK) return rc;
     const hipError_t err =
-        lz::launch_encode(sl->dev, n, symmetry, d_policy, d_value, s);
+        lz::launch_encode(sl->dev, n, symmetry, d_policy, d_value, sl->ovf, s);
     if (err != hipSuccess)
         return e->fail(LZ_ERR_HIP, "encoder launch: %s", hipGetErrorString(err));
     return release_slot(e, sl, s);

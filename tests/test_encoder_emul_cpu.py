@@ -166,3 +166,39 @@ def test_playout_kernel_matches_host_playouts(emul):
                        _p(ref, C.c_float))
     assert np.array_equal(out, ref), (out, ref)
     assert len(set(ref.tolist())) > 3  # the outcomes differ by position
+
+
+def lattice_moves(shift, extra=()):
+    """Moves of a 19x19 board whose stones are all single-stone strings
+    (black at x % 2 == shift, y even; white at the other parity, y odd:
+    180-181 strings), then `extra` moves; white passes when it runs out."""
+    v = lambda x, y: (y + 1) * 21 + (x + 1)  # noqa: E731
+    blk = [v(x, y) for y in range(0, 19, 2) for x in range(19)
+           if x % 2 == shift]
+    wht = [v(x, y) for y in range(1, 19, 2) for x in range(19)
+           if x % 2 != shift]
+    moves = []
+    for i, b in enumerate(blk):
+        moves += [b, wht[i] if i < len(wht) else -1]
+    return moves + list(extra)
+
+
+@pytest.mark.parametrize("net", [NET_POLICY, NET_VALUE])
+def test_kernel_many_strings(emul, net):
+    """Boards with more strings than the kernel keeps in LDS (128): the
+    rest go through its global overflow area; planes equal the host
+    encoder's."""
+    from leela_amd.positions import Position
+    v = lambda x, y: (y + 1) * 21 + (x + 1)  # noqa: E731
+    cases = [lattice_moves(0), lattice_moves(1),
+             lattice_moves(0, [v(1, 0), v(3, 0), v(0, 1), v(18, 17)]),
+             lattice_moves(1, [v(0, 0), v(1, 1), v(17, 17), v(8, 8)])]
+    for k, moves in enumerate(cases):
+        pos = Position(19)
+        for m in moves:
+            pos.play(m)
+        for sym in (0, 5):
+            got = encode_moves(emul, 19, moves, net, sym)
+            want = pos.planes(net, sym).reshape(32, 361)
+            assert np.array_equal(got, want), (k, sym,
+                                               np.argwhere(got != want)[:5])

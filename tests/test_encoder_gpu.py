@@ -132,3 +132,22 @@ def test_crafted_ladder_positions_match_host(enc):
     # These positions do read ladders: both ladder planes are set somewhere.
     pl = encode_batch(pos, NET_POLICY)
     assert pl[:, 25].sum() > 20 and pl[:, 26].sum() > 100
+
+
+def test_many_strings_overflow_matches_host(enc):
+    """Boards with more strings (180-181) than the kernel keeps in LDS
+    (128), mixed in one batch with generator positions: the strings past
+    the cap live in the encoder's global overflow area; planes equal the
+    host encoder's."""
+    from test_encoder_emul_cpu import lattice_moves
+    v = lambda x, y: (y + 1) * 21 + (x + 1)  # noqa: E731
+    many = [replay(19, lattice_moves(0)), replay(19, lattice_moves(1)),
+            replay(19, lattice_moves(0, [v(1, 0), v(3, 0), v(0, 1)])),
+            replay(19, lattice_moves(1, [v(0, 0), v(1, 1), v(8, 8)]))]
+    pos = []
+    for i in range(64):
+        pos.append(many[i % 4] if i % 3 == 0 else Position.random(500 + i))
+    for net in (NET_POLICY, NET_VALUE):
+        got = enc.encode(pos, net).cpu().numpy()
+        want = encode_batch(pos, net, 0, 8)
+        assert np.array_equal(got.reshape(want.shape), want)

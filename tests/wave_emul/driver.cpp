@@ -24,12 +24,15 @@ void run(const std::vector<GameState> &boards, int net, int sym,
         const int S = boards[i].board.side();
         off += (int64_t)lz::kPlanes * S * S;
     }
+    std::vector<uint64_t> ovf((size_t)lz::kEncStrOvf * lz::kEncStrWords *
+                              boards.size());
     if (out2)  // both layouts from one launch
-        lz::launch_encode(rec.data(), (int)rec.size(), sym, out, out2, nullptr);
+        lz::launch_encode(rec.data(), (int)rec.size(), sym, out, out2,
+                          ovf.data(), nullptr);
     else
         lz::launch_encode(rec.data(), (int)rec.size(), sym,
                           net == 0 ? out : nullptr, net == 1 ? out : nullptr,
-                          nullptr);
+                          ovf.data(), nullptr);
 }
 
 }  // namespace

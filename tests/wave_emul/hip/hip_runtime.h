@@ -28,6 +28,7 @@
 typedef int hipError_t;
 typedef void *hipStream_t;
 constexpr hipError_t hipSuccess = 0;
+constexpr hipError_t hipErrorInvalidValue = 1;
 inline hipError_t hipGetLastError() { return hipSuccess; }
 
 struct dim3 {
