@@ -23,7 +23,6 @@
 #include <chrono>
 #include <cmath>
 #include <memory>
-#include <mutex>
 #include <thread>
 #include <vector>
 
