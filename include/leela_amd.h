@@ -330,8 +330,10 @@ int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
  * with virtual loss on the path of each simulation in flight. With
  * LZ_EVAL_NETS every new leaf is expanded with the policy net (priors over
  * the Netresult's points that pass UCTNode::create_children's filter,
- * UCTNode.cpp:149-158, the 35 best as UCTNode.cpp:551-552 considers) and
- * valued with the value net, both through the
+ * UCTNode.cpp:149-158, the 35 best as UCTNode.cpp:551-552 considers; at
+ * the root the AVERAGE_ALL ensemble, eight rows in one batch, as
+ * netscore_children does there, UCTNode.cpp:103-106) and valued with the
+ * value net, both through the
  * synchronous facade calls, which share launches through the leaf queue
  * when one runs at `precision` (lz_queue_start); this replaces the
  * reference's one blocking forward per search thread and its

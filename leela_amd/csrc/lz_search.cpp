@@ -16,7 +16,8 @@
 // leaf's value with the sign alternating per ply. Children are the
 // Netresult's points that pass the filter of UCTNode::create_children
 // (UCTNode.cpp:149-158: not the ko square, not an own eye, not suicide),
-// the 35 with the highest priors, or a pass when none does. A position after two passes in a row is
+// the 35 with the highest priors, or a pass when none does; the root's
+// priors are the eight-symmetry average (AVERAGE_ALL). A position after two passes in a row is
 // scored by FastBoard::final_mc_score.
 #include <algorithm>
 #include <atomic>
@@ -60,7 +61,7 @@ public:
     virtual ~Evaluator() = default;
     // Appends (vertex, prior) for the candidate moves of st; false on a
     // backend error.
-    virtual bool expand(const go::GameState &st,
+    virtual bool expand(const go::GameState &st, bool at_root,
                         std::vector<std::pair<int, float>> &moves) = 0;
     virtual bool value(const go::GameState &st, uint32_t seed, float *v) = 0;
     // Children kept per expansion, by prior (0: all).
@@ -79,10 +80,13 @@ public:
     NetEvaluator(lz_ctx *ctx, int precision) : ctx_(ctx), precision_(precision) {}
     // A facade object per call: the threads share only the context (and
     // through it the leaf queue).
-    bool expand(const go::GameState &st,
+    // The root takes the eight-symmetry average (one batch of eight rows),
+    // as netscore_children does at the root (UCTNode.cpp:103-106).
+    bool expand(const go::GameState &st, bool at_root,
                 std::vector<std::pair<int, float>> &moves) override {
         Network net(ctx_, precision_);
-        const Network::Netresult r = net.get_scored_moves(st, Network::DIRECT, 0);
+        const Network::Netresult r = net.get_scored_moves(
+            st, at_root ? Network::AVERAGE_ALL : Network::DIRECT, 0);
         if (net.last_status() != LZ_OK) return false;
         for (const auto &sn : r)
             if (candidate(st, sn.second)) moves.emplace_back(sn.second, sn.first);
@@ -108,7 +112,7 @@ private:
 class PlayoutEvaluator : public Evaluator {
 public:
     explicit PlayoutEvaluator(int playouts) : playouts_(playouts) {}
-    bool expand(const go::GameState &st,
+    bool expand(const go::GameState &st, bool,
                 std::vector<std::pair<int, float>> &moves) override {
         const go::Board &b = st.board;
         for (int y = 0; y < b.side(); y++)
@@ -174,7 +178,7 @@ struct Search {
     bool expand_and_evaluate(Node &leaf, const go::GameState &st,
                              uint32_t seed, float *v_in) {
         std::vector<std::pair<int, float>> moves;
-        if (!eval.expand(st, moves)) return false;
+        if (!eval.expand(st, &leaf == &root, moves)) return false;
         policy_evals.fetch_add(1, std::memory_order_relaxed);
         float v_to_move = 0.5f;
         if (!eval.value(st, seed, &v_to_move)) return false;

@@ -47,8 +47,9 @@ def test_search_batched_through_queue(be):
         "simulations_per_s", "mean_batch", "collisions", "nodes")},
         flush=True, file=sys.stderr)
     check_tree(st, 2000)
-    # Each expansion is one policy row and one value row, all queued.
-    assert st["queue_positions"] == st["policy_evals"] + st["value_evals"]
+    # Each expansion is one policy row and one value row, all queued; the
+    # root's policy is the eight-symmetry average (seven rows more).
+    assert st["queue_positions"] == st["policy_evals"] + st["value_evals"] + 7
     assert st["mean_batch"] > 4.0
 
 
