@@ -482,8 +482,9 @@ def main() -> int:
                 "simulations_per_s_1_thread": round(
                     srch1["simulations_per_s"], 1),
                 "what": "lz_search: tree search with virtual loss, each new "
-                        "leaf one policy and one value row through the leaf "
-                        "queue (max batch 256); 1 thread: direct forwards",
+                        "leaf one policy row (async) and one value row "
+                        "through the leaf queue (max batch 256); 1 thread: "
+                        "direct forwards",
             },
             "playouts": {
                 "device_playouts_per_s": round(gpu_po_s, 1),

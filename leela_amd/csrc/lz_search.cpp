@@ -23,11 +23,14 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <future>
 #include <memory>
 #include <thread>
 #include <vector>
 
+#include "input_planes.h"
 #include "leela_amd.h"
+#include "lz_ctx.h"
 #include "lz_position.h"
 #include "network.h"
 #include "playout.h"
@@ -64,6 +67,13 @@ public:
     virtual bool expand(const go::GameState &st, bool at_root,
                         std::vector<std::pair<int, float>> &moves) = 0;
     virtual bool value(const go::GameState &st, uint32_t seed, float *v) = 0;
+    // Both for one leaf; by default one after the other.
+    virtual bool expand_value(const go::GameState &st, bool at_root,
+                              uint32_t seed,
+                              std::vector<std::pair<int, float>> &moves,
+                              float *v) {
+        return expand(st, at_root, moves) && value(st, seed, v);
+    }
     // Children kept per expansion, by prior (0: all).
     virtual size_t max_children() const { return 0; }
 };
@@ -97,11 +107,57 @@ public:
         *v = net.get_value(st, Network::DIRECT);
         return net.last_status() == LZ_OK;
     }
+    // With the leaf queue running at this precision, a leaf's policy row
+    // goes in asynchronously (lz_submit, as async_scored_moves does) while
+    // this thread runs its value row through the synchronous call: one
+    // queue round trip per leaf instead of two. The root keeps the
+    // sequential path (its AVERAGE_ALL policy).
+    bool expand_value(const go::GameState &st, bool at_root, uint32_t seed,
+                      std::vector<std::pair<int, float>> &moves,
+                      float *v) override {
+        if (at_root || queue_precision(ctx_) != precision_)
+            return Evaluator::expand_value(st, at_root, seed, moves, v);
+        Pending p;
+        p.st = &st;
+        std::vector<uint8_t> planes((size_t)kPlanes * 361);
+        gather_features_policy(st, planes.data());
+        p.ladder.assign(planes.begin() + 25 * 361, planes.begin() + 26 * 361);
+        std::future<int> done = p.done.get_future();
+        if (lz_submit(ctx_, LZ_NET_POLICY, planes.data(), policy_done, &p) !=
+            LZ_OK)
+            return false;
+        Network net(ctx_, precision_);
+        *v = net.get_value(st, Network::DIRECT);
+        const bool value_ok = net.last_status() == LZ_OK;
+        // The queue runs every submitted row's callback, failed or not.
+        if (done.get() != LZ_OK || !value_ok) return false;
+        for (const auto &sn : p.result)
+            if (candidate(st, sn.second)) moves.emplace_back(sn.second, sn.first);
+        return true;
+    }
     // The 35 best-scored children, as uct_select_child considers for a
     // node with net scores (UCTNode.cpp:551-552).
     size_t max_children() const override { return 35; }
 
 private:
+    struct Pending {
+        const go::GameState *st = nullptr;
+        std::vector<uint8_t> ladder;  // policy plane 25, board orientation
+        Network::Netresult result;
+        std::promise<int> done;
+    };
+    // On the queue's dispatcher thread: the Netresult of the row
+    // (Network.cpp:820-829, 656-667), then the waiting thread is released.
+    static void policy_done(void *user, int status, const float *,
+                            const float *out) {
+        Pending *p = static_cast<Pending *>(user);
+        if (status == LZ_OK) {
+            collect_scored_moves(*p->st, out, 0, p->result);
+            prune_ladders(*p->st, p->ladder.data(), p->result);
+        }
+        p->done.set_value(status);  // p is not touched after this
+    }
+
     lz_ctx *ctx_;
     int precision_;
 };
@@ -178,10 +234,10 @@ struct Search {
     bool expand_and_evaluate(Node &leaf, const go::GameState &st,
                              uint32_t seed, float *v_in) {
         std::vector<std::pair<int, float>> moves;
-        if (!eval.expand(st, &leaf == &root, moves)) return false;
-        policy_evals.fetch_add(1, std::memory_order_relaxed);
         float v_to_move = 0.5f;
-        if (!eval.value(st, seed, &v_to_move)) return false;
+        if (!eval.expand_value(st, &leaf == &root, seed, moves, &v_to_move))
+            return false;
+        policy_evals.fetch_add(1, std::memory_order_relaxed);
         value_evals.fetch_add(1, std::memory_order_relaxed);
         if (moves.empty()) moves.emplace_back(go::kPass, 1.0f);
         std::sort(moves.begin(), moves.end(),
