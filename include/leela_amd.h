@@ -333,9 +333,11 @@ int lz_network_async_scored_moves(lz_ctx *ctx, const lz_position *pos,
  * UCTNode.cpp:149-158, the 35 best as UCTNode.cpp:551-552 considers; at
  * the root the AVERAGE_ALL ensemble, eight rows in one batch, as
  * netscore_children does there, UCTNode.cpp:103-106) and valued with the
- * value net, both through the
- * synchronous facade calls, which share launches through the leaf queue
- * when one runs at `precision` (lz_queue_start); this replaces the
+ * value net. With the leaf queue running at `precision` (lz_queue_start)
+ * the policy row is submitted asynchronously (as async_scored_moves) while
+ * the thread runs the value row through the synchronous facade call, and
+ * the threads' rows share launches; without it each call is its own
+ * forward. This replaces the
  * reference's one blocking forward per search thread and its
  * thread_can_issue backpressure (OpenCL.cpp:446-454). LZ_EVAL_PLAYOUTS
  * (host only, ctx may be NULL) uses uniform priors and the mean of
