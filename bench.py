@@ -389,6 +389,10 @@ def main() -> int:
     s_threads, s_sims = 128, 4096
     be.queue_start(p, max_batch=256, max_wait_us=300)
     try:
+        # Warm-up search first (the queue's first launches of each bucket
+        # and the threads' first allocations), then the timed one.
+        lz_search.search(positions.Position.random(20), s_threads, 1024,
+                         backend=be, precision=p, seed=3)
         srch = lz_search.search(s_root, s_threads, s_sims, backend=be,
                                 precision=p, seed=4)
     finally:
