@@ -113,8 +113,9 @@ def main() -> int:
                          "byte counts")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: leaf positions are independent and "
-              "this build has no multi-GPU path (see DESIGN.md, sharding).",
+        print("bench.py measures one GPU only (--gpus 1): leaf positions are "
+              "independent, so more GPUs would each run their own share of "
+              "the batches; no multi-GPU path is implemented.",
               file=sys.stderr)
         return 2
 

@@ -21,7 +21,7 @@ float final_mc_score(const Board &b, float komi);
 // empty points that are not the ko square, not an own eye and not suicide
 // (pass when there is none). The reference draws moves from its pattern
 // policy (FastState::play_random_move, FastState.cpp:168-255); this light
-// policy is the build's teacher. Returns final_mc_score with the state's
+// policy is the surrogate's teacher here. Returns final_mc_score with the state's
 // komi.
 float playout(GameState &st, Rng &rng);
 

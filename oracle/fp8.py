@@ -1,6 +1,6 @@
 """FP8 e4m3 emulation of the library's FP8 form — TEST INFRASTRUCTURE ONLY.
 
-The FP8 form is this build's own variant (the reference computes in FP32),
+The FP8 form is this library's own variant (the reference computes in FP32),
 so this restates the form's definition, in FP64 arithmetic:
 
   * the 5x5 stem: weights rounded to FP16, inputs the binary planes;

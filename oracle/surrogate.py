@@ -1,6 +1,6 @@
 """FP64 CPU oracle of the learned playout surrogate — TEST INFRASTRUCTURE ONLY.
 
-The surrogate is this build's own network (the reference has none; its
+The surrogate is this library's own network (the reference has none; its
 teacher is Playout::mc_owner, Playout.cpp:174-217), so the oracle restates
 the network's definition, not a reference routine:
 
