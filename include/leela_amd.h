@@ -119,6 +119,23 @@ int lz_forward_timed(lz_ctx *ctx, int net, int precision, int n,
                      void *stream, float *launch_ms, int max_launches,
                      int *n_launches);
 
+/* As lz_forward_device for LZ_PREC_BF16 / LZ_PREC_FP16 / LZ_PREC_FP8 and
+ * 0 <= n <= LZ_NOALLOC_MAX_BOARDS boards of 19x19, with no allocation: no
+ * hipMalloc, hipFree or other allocating call, so it can be made where
+ * allocation is not allowed (code under HIP stream capture, tight serving
+ * loops, harnesses that forbid it). No counterpart in the reference; it is
+ * for those callers. The value net's head plane (n x 361 floats) goes to a
+ * static device array of LZ_NOALLOC_MAX_BOARDS x 361 floats (2.96 MB,
+ * held by each device the library runs on until the process ends) instead
+ * of the context's workspace. Every call on one device shares that array:
+ * calls that overlap on one GPU (other streams, other contexts) race, so
+ * the caller orders them. Larger n returns LZ_ERR_ARG; LZ_PREC_FP32 returns
+ * LZ_ERR_UNSUPPORTED (its activation maps need a workspace). */
+#define LZ_NOALLOC_MAX_BOARDS 2048
+int lz_forward_device_noalloc(lz_ctx *ctx, int net, int precision, int n,
+                              const uint8_t *d_planes, float *d_raw,
+                              float *d_out, void *stream);
+
 /* ---- Autotuner ----------------------------------------------------------
  * Times every launch shape of the fused kernels (wave grid 1x4 or 2x4;
  * persistent grid of CUs x occupancy x {1, 2} or one workgroup per slot)

@@ -23,6 +23,7 @@ LZ_ERR_HIP = -2
 LZ_ERR_STATE = -3
 LZ_ERR_NOMEM = -4
 LZ_ERR_UNSUPPORTED = -5
+NOALLOC_MAX_BOARDS = 2048
 
 NET_POLICY, NET_VALUE = 0, 1
 PREC_FP32, PREC_BF16, PREC_FP8, PREC_FP16 = 0, 1, 2, 3
@@ -74,6 +75,8 @@ def load_library():
                              C.POINTER(C.c_uint8), f32p, f32p]
     L.lz_forward_device.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
                                     vp]
+    L.lz_forward_device_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                                            vp, vp, vp, vp]
     L.lz_forward_timed.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
                                    vp, f32p, C.c_int, C.POINTER(C.c_int)]
     L.lz_forward_ragged.argtypes = [vp, C.c_int, C.c_int, C.c_int,
@@ -190,6 +193,15 @@ class Backend:
         self._check(self._L.lz_forward_device(self._ctx, net, precision, n,
                                               d_planes, d_raw, d_out, stream),
                     "lz_forward_device")
+
+    def forward_device_noalloc(self, net: int, precision: int, n: int,
+                               d_planes: int, d_raw: int, d_out: int,
+                               stream: int | None = None):
+        """lz_forward_device_noalloc: 16-bit / FP8 forms, n <= 2048, no
+        allocation (static scratch array)."""
+        self._check(self._L.lz_forward_device_noalloc(
+            self._ctx, net, precision, n, d_planes, d_raw, d_out, stream),
+            "lz_forward_device_noalloc")
 
     def forward_timed(self, net: int, precision: int, n: int, d_planes: int,
                       d_raw: int, d_out: int, stream: int | None = None):

@@ -91,6 +91,10 @@ int lz_init(lz_ctx **out, int device) {
                      prop.gcnArchName);
         return LZ_ERR_UNSUPPORTED;
     }
+    // The no-allocation entry point's scratch array: its address on this
+    // device, looked up once per device (the status is kept and reported
+    // by lz_forward_device_noalloc; other entry points do not need it).
+    (void)lz::noalloc_init(device);
     lz_ctx *ctx = new (std::nothrow) lz_ctx();
     if (!ctx) return LZ_ERR_NOMEM;
     ctx->device = device;

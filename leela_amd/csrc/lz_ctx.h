@@ -48,4 +48,7 @@ int forward_locked(lz_ctx *ctx, int net, int precision, int n,
 void destroy_queue(lz_ctx *ctx);
 // The running leaf queue's precision, -1 when none runs.
 int queue_precision(const lz_ctx *ctx);
+// Looks up, once per device, the address of the static scratch array of
+// lz_forward_device_noalloc on `device` (current); lz_noalloc.cpp.
+hipError_t noalloc_init(int device);
 }  // namespace lz
