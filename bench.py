@@ -3,11 +3,17 @@
 one MI355X, batch 1024 of 19x19 positions, inputs resident in HBM.
 
 A step is one policy forward plus one value forward over the batch (the two
-nets every leaf evaluation runs). The FP32 plain form, the BF16 form and the
-FP16-operand form run on the same positions and weights. The headline
-`value` is the fastest 16-bit form whose policy top-1 agrees with the FP32
-plain form (itself within 1e-4 of the reference CPU path) on >= 99.5% of the
-batch; every form's figures are in `config`. Prints one JSON line.
+nets every leaf evaluation runs). The FP32 plain form, the BF16 form, the
+FP16-operand form and the FP8 form run on the same positions and weights.
+The CPU-baseline leg times the reference CPU path (the oracle's restatement)
+and computes the FP64 oracle's outputs for the batch, against which every
+form's top-1 agreement, KL, max-abs policy error and value error are
+reported. The headline `value` is the fastest 16-bit form whose policy
+top-1 agrees with the FP64 oracle on >= 99.5% of the batch; the process
+exits with status 3 (after printing) when no form meets that bar. Also
+reported: the ragged mixed-board workload (1024 x 19x19 + 512 x 13x13 +
+512 x 9x9 in one launch, autotuned, and through the leaf queue from 16
+threads). Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -28,8 +34,9 @@ AGREEMENT_BAR = 0.995
 # HBM bytes per launch of the roofline kernel at batch 1024, measured with
 # rocprofv3 FETCH_SIZE / WRITE_SIZE (one counter per run) and scaled by the
 # counters' reading for a known-size elementwise copy (reads x2.0, writes
-# x1.0); the summary with the commands is
-# profiles/fused_net_policy_hbm_traffic_b1024.md. Keyed by form.
+# x1.0). Not measured in this run: the figures, their commands and the
+# build they were measured on are in TRAFFIC_SOURCE. Keyed by form.
+TRAFFIC_SOURCE = "profiles/fused_net_policy_hbm_traffic_b1024.md"
 TRAFFIC_B1024 = {
     "fp16": {"bytes": 82_946_000, "fetch_kib": 39029.2, "write_kib": 2944.0},
     "bf16": {"bytes": 52_970_000, "fetch_kib": 24390.1, "write_kib": 2944.0},
@@ -47,6 +54,45 @@ def _cpu_model() -> str:
     except OSError:
         pass
     return "unknown"
+
+
+def host_cores() -> int:
+    """Cores this process may run on (its CPU affinity)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def fp64_reference(pol, val, planes, vplanes, threads: int):
+    """The FP64 oracle's outputs for the batch (CPU-baseline leg: the
+    oracle as the checker of every form's accuracy)."""
+    import oracle
+    oracle.load_blas()
+    t0 = time.perf_counter()
+    p = oracle.forward_f64(oracle.Net(pol), planes, oracle.HEAD_POLICY,
+                           threads=threads, gemm=True)
+    v = oracle.forward_f64(oracle.Net(val), vplanes, oracle.HEAD_VALUE,
+                           threads=threads, gemm=True)
+    return p, v, time.perf_counter() - t0
+
+
+def accuracy_vs_fp64(logits, probs, win, ref):
+    """Top-1 agreement, mean KL(FP64 || form), max |d prob|, max |d logit|
+    and max |d win rate| against the FP64 oracle."""
+    (raw64, p64), (_, w64) = ref
+    q = np.maximum(probs.astype(np.float64), 1e-30)
+    p = np.maximum(p64, 1e-30)
+    kl = np.sum(p * (np.log(p) - np.log(q)), axis=1)
+    return {"top1_vs_fp64": round(float(np.mean(logits.argmax(1) ==
+                                                raw64.argmax(1))), 5),
+            "mean_kl_vs_fp64": float(f"{kl.mean():.4g}"),
+            "max_abs_policy_err_vs_fp64": float(
+                f"{np.abs(probs - p64).max():.4g}"),
+            "max_abs_logit_err_vs_fp64": float(
+                f"{np.abs(logits - raw64).max():.4g}"),
+            "max_abs_value_err_vs_fp64": float(
+                f"{np.abs(win.ravel() - w64.ravel()).max():.4g}")}
 
 
 def cpu_baseline(pol, val, planes, vplanes, budget_s: float, threads: int):
@@ -83,11 +129,93 @@ def cpu_baseline(pol, val, planes, vplanes, budget_s: float, threads: int):
     return {"value": round(v, 2), "unit": "positions/s", "cores": threads,
             "kind": "port",
             "sample": f"{m} of the benchmark's positions, policy+value, "
-                      f"{threads} worker threads x 1 BLAS thread, {blas}",
+                      f"{threads} worker threads x 1 BLAS thread (every "
+                      f"core this process may use), {blas}",
             "value_1_thread": round(v1, 2), "sample_1_thread": m1,
             "with_encoder": round(ve, 2),
             "with_encoder_1_thread": round(ve1, 2),
+            "host_cores_available": host_cores(),
+            "host_cpus_online": os.cpu_count(),
             "cpu_model": _cpu_model()}
+
+
+def ragged_leg(be, leela_amd, nets, positions, p, stream, dev, torch,
+               steps):
+    """The mixed-board workload (1024 x 19x19 + 512 x 13x13 + 512 x 9x9):
+    one ragged launch per net after autotuning the launch shape for this
+    mix, timed with HIP events on the benchmark stream; then the same
+    boards through the leaf queue from 16 threads (max batch 2048)."""
+    if p not in (leela_amd.PREC_BF16, leela_amd.PREC_FP16):
+        return None
+    sides, pol, val = positions.ragged_mix()
+    n = len(sides)
+    tune = be.autotune_ragged(p, sides)
+    chosen = [ln for ln in tune.splitlines() if "chosen" in ln]
+    pp = torch.from_numpy(np.concatenate([b.ravel() for b in pol])).to(dev)
+    vp = torch.from_numpy(np.concatenate([b.ravel() for b in val])).to(dev)
+    pts = sum(s * s for s in sides)
+    praw = torch.empty(pts, dtype=torch.float32, device=dev)
+    pout = torch.empty_like(praw)
+    vraw = torch.empty(n, dtype=torch.float32, device=dev)
+    vout = torch.empty_like(vraw)
+    ext = torch.cuda.ExternalStream(stream)
+
+    def rstep():
+        be.forward_ragged_device(leela_amd.NET_POLICY, p, sides,
+                                 pp.data_ptr(), praw.data_ptr(),
+                                 pout.data_ptr(), stream)
+        be.forward_ragged_device(leela_amd.NET_VALUE, p, sides,
+                                 vp.data_ptr(), vraw.data_ptr(),
+                                 vout.data_ptr(), stream)
+
+    for _ in range(3):
+        rstep()
+    torch.cuda.synchronize(dev)
+    k = max(5, steps // 2)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * k)]
+    t0 = time.perf_counter()
+    for i in range(k):
+        ev[2 * i].record(ext)
+        rstep()
+        ev[2 * i + 1].record(ext)
+    torch.cuda.synchronize(dev)
+    wall = (time.perf_counter() - t0) / k
+    dev_ms = float(np.median([ev[2 * i].elapsed_time(ev[2 * i + 1])
+                              for i in range(k)]))
+    conv = sum(2 * co * ci * f * f for kd, f, ci, co in
+               nets.POLICY_SHAPES + nets.VALUE_SHAPES if kd == nets.CONV)
+    fc = sum(2 * co * ci for kd, f, ci, co in nets.VALUE_SHAPES
+             if kd == nets.FC)
+    flops = conv * pts + fc * n
+    tf = flops / (dev_ms * 1e-3) / 1e12
+
+    be.queue_start(p, max_batch=2048, max_wait_us=2000)
+    try:
+        be.queue_benchmark(sides, pol, val, threads=16, rounds=1)
+        st0 = be.queue_stats()
+        q_pos_s = be.queue_benchmark(sides, pol, val, threads=16, rounds=4)
+        st1 = be.queue_stats()
+    finally:
+        be.queue_stop()
+    return {
+        "workload": "1024 x 19x19 + 512 x 13x13 + 512 x 9x9 generator "
+                    "positions (move counts uniform in 0..0.6 x area), "
+                    "policy + value, one ragged launch per net",
+        "boards": n,
+        "positions_per_s": round(n / (dev_ms * 1e-3), 1),
+        "positions_per_s_wall": round(n / wall, 1),
+        "ms_per_step": round(dev_ms, 4),
+        "tflops": round(tf, 2),
+        "frac_of_peak": round(tf / 2500.0, 4),
+        "flops_per_step": flops,
+        "flops_rule": "conv layers x board points (P/361 of a 19x19 "
+                      "board's), value inner products per board",
+        "autotune": chosen,
+        "queue_16_threads_positions_per_s": round(q_pos_s, 1),
+        "queue_launches_per_bucket": {
+            str(b): int(st1["per_bucket"][b] - st0["per_bucket"][b])
+            for b in (64, 256, 1024, 2048)},
+    }
 
 
 def main() -> int:
@@ -175,9 +303,10 @@ def main() -> int:
         torch.cuda.synchronize(dev)
         dt = time.perf_counter() - t0
         logits = d_praw.cpu().numpy().copy()
+        probs = d_pout.cpu().numpy().copy()
         win = d_vout.cpu().numpy().copy()
         return {"dt": dt, "steps": steps, "pos_s": B * steps / dt,
-                "logits": logits, "win": win}
+                "logits": logits, "probs": probs, "win": win}
 
     if args.kernel_only:
         if args.kernel_only == "encoder":
@@ -233,10 +362,29 @@ def main() -> int:
                                   ref["logits"].argmax(1)))
         r["dwin"] = float(np.abs(r["win"] - ref["win"]).max())
 
+    # CPU-baseline leg, part 1: the FP64 oracle's outputs for the batch, the
+    # checker of every form's agreement (without it, agreement is judged
+    # against the FP32 plain form, itself within 1e-4 of the CPU path).
+    cores = host_cores()
+    ref64 = None
+    fp64_s = None
+    if not args.no_cpu_baseline:
+        p64, v64, fp64_s = fp64_reference(pol, val, planes, vplanes, cores)
+        ref64 = (p64, v64)
+        for name in forms:
+            r = res[name]
+            r["acc64"] = accuracy_vs_fp64(r["logits"], r["probs"], r["win"],
+                                          ref64)
+            r["agree"] = r["acc64"]["top1_vs_fp64"]
+    else:
+        for name in forms:
+            res[name]["agree"] = res[name]["top1"]
+
     head = args.precision
+    ok = [n for n in ("bf16", "fp16") if res[n]["agree"] >= AGREEMENT_BAR]
     if head == "auto":
-        ok = [n for n in ("bf16", "fp16") if res[n]["top1"] >= AGREEMENT_BAR]
         head = max(ok, key=lambda n: res[n]["pos_s"]) if ok else "bf16"
+    agreement_met = head in ok
     if head != "fp32" and res[head]["steps"] != args.steps:
         res[head] = {**res[head], **run(head, args.steps, args.warmup)}
     h = res[head]
@@ -400,16 +548,34 @@ def main() -> int:
         be.queue_stop()
     srch1 = lz_search.search(s_root, 1, 256, backend=be, precision=p, seed=4)
 
+    # Config 3: ragged batched leaf evaluation, 1024 x 19x19 + 512 x 13x13
+    # + 512 x 9x9 (random legal play, move counts uniform in 0..0.6 x board
+    # area, one seed range per side) in one persistent launch per net, the
+    # launch shape set by the autotuner for this mix; then the same boards
+    # through the leaf queue from 16 threads (ragged graph of the 2048
+    # bucket). FLOPs: the conv layers scale with the board's points
+    # (P / 361); the value head's inner products see every board as 361
+    # inputs (smaller boards embedded at y*19+x), so they do not.
+    ragged = ragged_leg(be, leela_amd, nets, positions, p, stream, dev,
+                        torch, args.steps)
+
     # Dominant kernel, timed per launch with HIP events on its stream.
-    kern_ms = []
-    settle(p)
-    for _ in range(20):
-        ms = be.forward_timed(leela_amd.NET_POLICY, p, B, d_planes.data_ptr(),
-                              d_praw.data_ptr(), d_pout.data_ptr(), stream)
-        kern_ms += list(ms[2:12]) if head == "fp32" else list(ms)
-    avg_ms = float(np.mean(kern_ms))
+    def kernel_ms(form, sel):
+        q = leela_amd.PRECISIONS[form]
+        settle(q)
+        out = []
+        for _ in range(20):
+            ms = be.forward_timed(leela_amd.NET_POLICY, q, B,
+                                  d_planes.data_ptr(), d_praw.data_ptr(),
+                                  d_pout.data_ptr(), stream)
+            out += list(sel(ms))
+        return float(np.mean(out))
+
+    trunk_flops = 2.0 * 128 * 1152 * 361 * B  # one trunk layer, whole batch
+    avg_ms = kernel_ms(head, (lambda ms: ms[2:12]) if head == "fp32"
+                       else (lambda ms: ms))
     if head == "fp32":
-        kflops = 2.0 * 128 * 1152 * 361 * B  # one trunk layer, whole batch
+        kflops = trunk_flops
         kernel = "conv_f32_kernel<3> (policy trunk 3x3 128->128)"
     else:
         kflops = float(nets.flops_per_position(nets.POLICY_SHAPES)) * B
@@ -419,19 +585,37 @@ def main() -> int:
     w_bytes = 1664609 * (4 if head == "fp32" else 2)
     algo_bytes = B * (32 * 361 + 2 * 4 * 361) + w_bytes
     peak = PEAK_TFLOPS[head]
+    # The FP32 plain form's dominant kernel (the ten 128->128 trunk convs of
+    # the policy net, launches 2..11 of its forward) against the FP32
+    # matrix peak.
+    f32_ms = kernel_ms("fp32", lambda ms: ms[2:12])
+    f32_tf = trunk_flops / (f32_ms * 1e-3) / 1e12
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(pol, val, planes, vplanes, args.cpu_seconds,
-                           threads)
+                           cores)
+        cpu["fp64_oracle_seconds"] = round(fp64_s, 2)
 
-    forms_out = {
-        n: {"positions_per_s": round(res[n]["pos_s"], 1),
+    step_flops = float(nets.flops_per_position(nets.POLICY_SHAPES) +
+                       nets.flops_per_position(nets.VALUE_SHAPES))
+    forms_out = {}
+    for n in forms:
+        tf = res[n]["pos_s"] * step_flops / 1e12
+        forms_out[n] = {
+            "positions_per_s": round(res[n]["pos_s"], 1),
             "speedup_vs_fp32_plain": round(res[n]["pos_s"] / ref["pos_s"], 3),
+            "step_tflops": round(tf, 2),
+            "step_frac_of_peak": round(tf / PEAK_TFLOPS[n], 4),
             "top1_agreement_vs_fp32_plain": round(res[n]["top1"], 5),
-            "max_abs_value_diff_vs_fp32_plain": round(res[n]["dwin"], 6)}
-        for n in forms}
+            "max_abs_value_diff_vs_fp32_plain": round(res[n]["dwin"], 6),
+            **res[n].get("acc64", {})}
+    forms_out["fp32"]["roofline"] = {
+        "kernel": "conv_f32_kernel<3> (policy trunk 3x3 128->128, ten "
+                  "launches per forward)",
+        "achieved": round(f32_tf, 2), "peak": PEAK_TFLOPS["fp32"],
+        "unit": "TFLOP/s", "frac": round(f32_tf / PEAK_TFLOPS["fp32"], 4),
+        "avg_launch_ms": round(f32_ms, 4)}
     line = {
         "metric": "NN positions/sec, BF16 vs FP32 plain, batch 1024 at "
                   ">=99.5% top-1 agreement",
@@ -456,7 +640,11 @@ def main() -> int:
                          "counts uniform in 0..250",
             "parallelism": "single GPU",
             "agreement_bar": AGREEMENT_BAR,
+            "agreement_reference": ("FP64 oracle (im2col + dgemm)"
+                                    if ref64 else "FP32 plain form"),
+            "agreement_met": agreement_met,
             "forms": forms_out,
+            "ragged": ragged,
             "host_buffer_positions_per_s": round(host_pos_s, 1),
             "graph_replay_positions_per_s": (round(graph_pos_s, 1)
                                              if graph_pos_s else None),
@@ -507,10 +695,12 @@ def main() -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": (TRAFFIC_B1024[head]["bytes"] if B == 1024 else None),
+            "traffic": (TRAFFIC_B1024[head]["bytes"]
+                        if B == 1024 and head in TRAFFIC_B1024 else None),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + "
                             "WRITE_SIZE x1.0, calibrated on a known-size "
                             "copy)",
+            "traffic_source": TRAFFIC_SOURCE,
             "avg_launch_ms": round(avg_ms, 4),
             # Planes in + logits and probabilities out per board, weights
             # once per launch (16-bit), over the launch time.
@@ -521,6 +711,10 @@ def main() -> int:
     }
     print(json.dumps(line))
     be.close()
+    if not agreement_met:
+        print(f"no 16-bit form meets the {AGREEMENT_BAR} top-1 agreement "
+              f"bar", file=sys.stderr)
+        return 3
     return 0
 
 

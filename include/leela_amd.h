@@ -146,23 +146,39 @@ int lz_forward_device_noalloc(lz_ctx *ctx, int net, int precision, int n,
  * it in a text file. */
 int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
                 size_t report_len);
+/* The same for a ragged batch of n boards of sides[0..n) (5..19): times
+ * lz_forward_ragged's launches of every shape and keeps the fastest for
+ * the bucket of the batch's slot count (config: 1024 x 19x19 + 512 x
+ * 13x13 + 512 x 9x9 packs into 1408 slots). */
+int lz_autotune_ragged(lz_ctx *ctx, int precision, int n,
+                       const int32_t *sides, char *report,
+                       size_t report_len);
 int lz_autotune_save(lz_ctx *ctx, const char *path);
 int lz_autotune_load(lz_ctx *ctx, const char *path);
 
 /* ---- Batched leaf queue -------------------------------------------------
  * Replaces the per-thread asynchronous path: Network::async_scored_moves
  * -> OpenCL_Network::forward with forward_cb (Network.cpp:471-587,
- * OpenCL.cpp:490-569), which evaluates one position per forward per search
- * thread. Any number of threads submit single positions; a dispatcher
- * thread batches up to max_batch of one net (or whatever has waited
- * max_wait_us) and replays a hipGraph captured for the smallest bucket of
- * 64 / 256 / 1024 / 2048 boards that holds the batch. The softmax
- * temperature in effect at lz_queue_start is baked into the graphs. */
+ * OpenCL.cpp:490-569, per-thread queues OpenCL.cpp:446-472), which
+ * evaluates one position per forward per search thread. Any number of
+ * threads submit single positions; they are written straight into one of
+ * four batch buffers. A dispatcher thread closes a batch at max_batch rows
+ * of one net (or when its oldest row has waited max_wait_us, or on flush)
+ * and replays the hipGraph captured for the smallest bucket of 64 / 256 /
+ * 1024 / 2048 boards that holds it: the dense graph when every board is
+ * 19x19, the ragged graph (mixed sides packed into slots, LZ_PREC_BF16 /
+ * LZ_PREC_FP16) otherwise. The 16-bit and FP8 graphs evaluate exactly the
+ * batch's rows; the FP32 graphs the whole bucket. While the device works
+ * the dispatcher fills and launches the next batch; a completion thread
+ * runs the callbacks. The softmax temperature in effect at lz_queue_start
+ * is baked into the graphs. */
 
-/* Completion callback, run on the dispatcher thread. raw/out point at the
- * position's results (policy: 361 floats each; value: 1 float each) and are
- * valid only during the call; status != LZ_OK means the batch failed and
- * raw/out are NULL. */
+/* Completion callback, run on the queue's completion thread (never inside
+ * the HIP runtime). raw/out point at the position's results (policy: S*S
+ * floats for a board of side S; value: 1 float) and are valid only during
+ * the call; status != LZ_OK means the batch failed and raw/out are NULL. A
+ * callback may submit, but when every batch buffer is busy such a submit
+ * returns LZ_ERR_STATE instead of waiting for one. */
 typedef void (*lz_done_fn)(void *user, int status, const float *raw,
                            const float *out);
 
@@ -170,9 +186,29 @@ int lz_queue_start(lz_ctx *ctx, int precision, int max_batch,
                    int max_wait_us);
 
 /* Copies planes (uint8 [32][361]) into the queue; `done` runs once with
- * the result. Thread-safe. */
+ * the result. Thread-safe; waits while every batch buffer is busy. */
 int lz_submit(lz_ctx *ctx, int net, const uint8_t *planes, lz_done_fn done,
               void *user);
+
+/* As lz_submit for a board of side 5..19 (planes uint8 [32][side*side],
+ * as lz_encode_planes writes them). Boards of different sides share
+ * ragged launches; a queue at LZ_PREC_FP32 or LZ_PREC_FP8 takes 19x19 only
+ * (LZ_ERR_UNSUPPORTED). The reference evaluates 19x19 only
+ * (Network.cpp:541-543); smaller boards follow lz_forward_ragged. */
+int lz_submit_side(lz_ctx *ctx, int net, int side, const uint8_t *planes,
+                   lz_done_fn done, void *user);
+
+/* Measurement hook (no counterpart in the reference): `threads` host
+ * threads submit the n boards (sides[i] in 5..19; planes packed board after
+ * board, uint8 [32][S_i*S_i] each, as lz_forward_ragged takes them) to the
+ * running queue, thread t boards t, t + threads, ..., each board's policy
+ * row then its value row, `rounds` times over; then lz_join. Writes
+ * rounds * n / elapsed seconds: positions evaluated by both nets per
+ * second, from the first submit to the last callback. */
+int lz_queue_benchmark(lz_ctx *ctx, int n, const int32_t *sides,
+                       const uint8_t *policy_planes,
+                       const uint8_t *value_planes, int threads, int rounds,
+                       double *positions_per_s);
 
 /* Launches what is queued without waiting for a full batch. */
 int lz_flush(lz_ctx *ctx);
@@ -185,7 +221,8 @@ int lz_join(lz_ctx *ctx);
  * backpressure test (OpenCL::thread_can_issue, OpenCL.cpp:446-454). */
 int lz_queue_depth(lz_ctx *ctx);
 
-/* stats[6] = {batches, positions, batches per bucket 64/256/1024/2048}. */
+/* stats[6] = {batches launched, positions, batches per bucket 64 / 256 /
+ * 1024 / 2048}. */
 int lz_queue_stats(lz_ctx *ctx, int64_t *stats);
 
 /* Drains and stops the queue (also done by lz_destroy). */

@@ -86,10 +86,19 @@ def load_library():
                                            C.POINTER(C.c_int32), vp, vp, vp,
                                            vp]
     L.lz_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+    L.lz_autotune_ragged.argtypes = [vp, C.c_int, C.c_int,
+                                     C.POINTER(C.c_int32), C.c_char_p,
+                                     C.c_size_t]
+    L.lz_queue_benchmark.argtypes = [vp, C.c_int, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_uint8),
+                                     C.POINTER(C.c_uint8), C.c_int, C.c_int,
+                                     C.POINTER(C.c_double)]
     L.lz_autotune_save.argtypes = [vp, C.c_char_p]
     L.lz_autotune_load.argtypes = [vp, C.c_char_p]
     L.lz_queue_start.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.lz_submit.argtypes = [vp, C.c_int, C.POINTER(C.c_uint8), DONE_FN, vp]
+    L.lz_submit_side.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_uint8),
+                                 DONE_FN, vp]
     L.lz_flush.argtypes = [vp]
     L.lz_join.argtypes = [vp]
     L.lz_queue_depth.argtypes = [vp]
@@ -248,6 +257,33 @@ class Backend:
                                         len(buf)), "lz_autotune")
         return buf.value.decode()
 
+    def autotune_ragged(self, precision: int, sides) -> str:
+        """autotune for a ragged batch of boards of these sides."""
+        sides = np.ascontiguousarray(sides, np.int32)
+        buf = C.create_string_buffer(8192)
+        self._check(self._L.lz_autotune_ragged(
+            self._ctx, precision, len(sides), _ptr(sides, C.c_int32), buf,
+            len(buf)), "lz_autotune_ragged")
+        return buf.value.decode()
+
+    def queue_benchmark(self, sides, policy_boards, value_boards,
+                        threads: int = 16, rounds: int = 1) -> float:
+        """lz_queue_benchmark over boards (lists of [32][S*S] arrays);
+        returns positions per second (both nets)."""
+        sides = np.ascontiguousarray(sides, np.int32)
+        pp = np.concatenate([np.ascontiguousarray(b, np.uint8).ravel()
+                             for b in policy_boards])
+        vp_ = np.concatenate([np.ascontiguousarray(b, np.uint8).ravel()
+                              for b in value_boards])
+        out = C.c_double(0.0)
+        rc = self._L.lz_queue_benchmark(
+            self._ctx, len(sides), _ptr(sides, C.c_int32),
+            _ptr(pp, C.c_uint8), _ptr(vp_, C.c_uint8), threads, rounds,
+            C.byref(out))
+        if rc != LZ_OK:
+            raise LeelaError(rc, "lz_queue_benchmark")
+        return out.value
+
     def autotune_save(self, path: str):
         self._check(self._L.lz_autotune_save(self._ctx, path.encode()),
                     "lz_autotune_save")
@@ -272,6 +308,16 @@ class Backend:
                                C.c_void_p(user))
         if rc != LZ_OK:
             raise LeelaError(rc, "lz_submit")
+
+    def submit_side(self, net: int, side: int, planes: np.ndarray, done,
+                    user: int = 0):
+        """A board of any side 5..19: planes uint8 [32][side*side]."""
+        planes = np.ascontiguousarray(planes, dtype=np.uint8)
+        rc = self._L.lz_submit_side(self._ctx, net, side,
+                                    _ptr(planes, C.c_uint8), done,
+                                    C.c_void_p(user))
+        if rc != LZ_OK:
+            raise LeelaError(rc, "lz_submit_side")
 
     def flush(self):
         self._check(self._L.lz_flush(self._ctx), "lz_flush")

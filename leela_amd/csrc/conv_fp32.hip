@@ -246,10 +246,15 @@ __global__ __launch_bounds__(kThreads) void value_fc_kernel(
     const float *__restrict__ h0, const float *__restrict__ w1t,
     const float *__restrict__ b1, const float *__restrict__ w2,
     const float *__restrict__ b2, float *__restrict__ raw,
-    float *__restrict__ out, int n, int P, int H) {
+    float *__restrict__ out, int n, int P, int H,
+    const int *__restrict__ n_dev) {
     __shared__ float s_h[kFcBoards][kMaxSide * kMaxSide];
     __shared__ float s_red[kFcBoards][kThreads / 64];
+    // A replayed graph sizes its grid for the bucket; the batch's board
+    // count is then read from device memory.
+    if (n_dev) n = min(n, *n_dev);
     const int b0 = blockIdx.x * kFcBoards, tid = threadIdx.x;
+    if (b0 >= n) return;
     const int nb = min(kFcBoards, n - b0);
     for (int i = tid; i < kFcBoards * P; i += kThreads) {
         const int bb = i / P, p = i % P;
@@ -343,14 +348,15 @@ hipError_t value_headconv_f32(const float *in, const float *w,
 hipError_t value_fc_head(const float *h0, const float *fc1_wt,
                          const float *fc1_b, const float *fc2_w,
                          const float *fc2_b, float *raw, float *out, int n,
-                         int P, int hidden, hipStream_t stream) {
+                         int P, int hidden, hipStream_t stream,
+                         const int *n_dev) {
     if (n <= 0) return hipSuccess;
     if (P < 1 || P > kMaxSide * kMaxSide || hidden < 1)
         return hipErrorInvalidValue;
     const int grid = (n + kFcBoards - 1) / kFcBoards;
     hipLaunchKernelGGL(value_fc_kernel, dim3(grid), dim3(kThreads), 0, stream,
                        h0, fc1_wt, fc1_b, fc2_w, fc2_b, raw, out, n, P,
-                       hidden);
+                       hidden, n_dev);
     return hipGetLastError();
 }
 

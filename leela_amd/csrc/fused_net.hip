@@ -428,6 +428,7 @@ __global__ __launch_bounds__(kWM * kWN * 64,
     __shared__ float s_red[kWaves];
     T *s_stem = reinterpret_cast<T *>(s_raw);
     uint8_t *s_t8 = s_raw;
+    if (args.counts) nslots = min(nslots, args.counts[0]);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / kWN, wn = wave % kWN;
@@ -560,6 +561,7 @@ __global__ __launch_bounds__(kWM * kWN * 64, OCC) void fused_net(
     const int wm = wave / kWN, wn = wave % kWN;
     const int nt0 = wn * kNT;
     auto W = [&](int l) { return static_cast<const v8 *>(args.w[l]); };
+    if (args.counts) nslots = min(nslots, args.counts[0]);
 
     for (int slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
         // Slot geometry: k boards of side S stacked in the tile.
@@ -717,7 +719,7 @@ hipError_t launch_wm(int which, const uint8_t *planes, const FusedArgs &args,
     if (e != hipSuccess) return e;
     return value_fc_head(args.h0, args.fc1_wt, args.fc1_b, args.fc2_w,
                          args.fc2_b, raw, out, nboards, kP, args.hidden,
-                         stream);
+                         stream, args.counts ? args.counts + 1 : nullptr);
 }
 
 template <int WM>
@@ -741,7 +743,8 @@ hipError_t launch_f8_wm(int which, const uint8_t *planes, const FusedArgs &args,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return value_fc_head(args.h0, args.fc1_wt, args.fc1_b, args.fc2_w,
-                         args.fc2_b, raw, out, n, kP, args.hidden, stream);
+                         args.fc2_b, raw, out, n, kP, args.hidden, stream,
+                         args.counts ? args.counts + 1 : nullptr);
 }
 
 template <typename T, bool RAGGED>

@@ -40,6 +40,10 @@ struct FusedArgs {
     int nslots = 0;
     const int *in_off = nullptr;    // board -> byte offset of its planes
     const int *out_off = nullptr;   // board -> offset of its policy outputs
+    // Device-side {slots, boards} of the batch, read when the kernels start
+    // (nullptr: the launch's own counts). A graph captured for a bucket's
+    // capacity then evaluates exactly the batch it is replayed on.
+    const int *counts = nullptr;
     int hidden = 0;
     int nconv = 0;
     int c = 0;   // trunk width

@@ -32,10 +32,13 @@ hipError_t value_headconv_f32(const float *in, const float *w, const float *b,
 
 // Value inner-product head over a batch: FC (P -> hidden) + ELU, FC
 // (hidden -> 1), (1 + tanh) / 2. fc1_wt is the first FC transposed:
-// [P][hidden]. Shared by every precision (the FCs stay FP32).
+// [P][hidden]. Shared by every precision (the FCs stay FP32). n_dev (if not
+// null) is a device-side board count that caps n when the kernel runs (a
+// graph captured for n boards replayed on fewer).
 hipError_t value_fc_head(const float *h0, const float *fc1_wt,
                          const float *fc1_b, const float *fc2_w,
                          const float *fc2_b, float *raw, float *out, int n,
-                         int P, int hidden, hipStream_t stream);
+                         int P, int hidden, hipStream_t stream,
+                         const int *n_dev = nullptr);
 
 }  // namespace lz

@@ -44,6 +44,14 @@ int validate_net(Net &net, int which, std::string *msg) {
     net.head = nconv - 1;
     net.side = 19;
     net.max_ch = L[0].cin;
+    // Every path that feeds the nets (lz_forward's staging, the leaf queue,
+    // the facade, the encoders) lays out the reference's 32 input planes
+    // per board (POLICY_CHANNELS = VALUE_CHANNELS = 32, Network.h:48-49).
+    if (L[0].cin != 32) {
+        *msg = "the first conv must take the 32 input planes (got " +
+               std::to_string(L[0].cin) + ")";
+        return LZ_ERR_UNSUPPORTED;
+    }
     for (int k = 0; k < nconv; k++) {
         if (k > 0 && L[k].cin != L[k - 1].cout) {
             *msg = "conv " + std::to_string(k) + " input channels do not "
@@ -394,7 +402,7 @@ int run_net(const Net &net, int which, int precision, int n,
             const uint8_t *d_planes, float *d_raw, float *d_out,
             float temperature, Workspace &ws, hipStream_t stream,
             std::string *msg, LaunchTimer *timer,
-            const RaggedLayout *ragged) {
+            const RaggedLayout *ragged, const int *counts) {
     if (ragged && precision != LZ_PREC_BF16 && precision != LZ_PREC_FP16) {
         *msg = "ragged batches run on the 16-bit fused kernels only";
         return LZ_ERR_UNSUPPORTED;
@@ -410,6 +418,7 @@ int run_net(const Net &net, int which, int precision, int n,
         hipError_t e = timer ? timer->mark(stream) : hipSuccess;
         FusedArgs args = net.fused8;
         args.h0 = ws.h0;
+        args.counts = counts;
         // Default wave grid, one workgroup per board (the FP8 tile is
         // smaller than the 16-bit one, so occupancy differs from theirs).
         FusedLaunch cfg;
@@ -433,6 +442,7 @@ int run_net(const Net &net, int which, int precision, int n,
         hipError_t e = timer ? timer->mark(stream) : hipSuccess;
         FusedArgs args = bf ? net.fused : net.fused16;
         args.h0 = ws.h0;
+        args.counts = counts;
         if (ragged) {
             args.slots = ragged->d_slots;
             args.nslots = ragged->nslots;

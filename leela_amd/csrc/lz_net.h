@@ -104,7 +104,12 @@ int run_net(const Net &net, int which, int precision, int n,
             const uint8_t *d_planes, float *d_raw, float *d_out,
             float temperature, Workspace &ws, hipStream_t stream,
             std::string *msg, LaunchTimer *timer = nullptr,
-            const RaggedLayout *ragged = nullptr);
+            const RaggedLayout *ragged = nullptr,
+            const int *counts = nullptr);
+// counts: device-side {slots, boards} the fused forms read when they start
+// (graph replays of a bucket on a smaller batch); n (and the ragged
+// layout's nslots) are then capacities. The FP32 form ignores it and
+// evaluates all n rows.
 
 // Host packing of a conv into MFMA fragment order (16-bit values, round to
 // nearest even): [k-step][m-tile][lane][8], k-step s = tap * (cin/32) +

@@ -21,10 +21,12 @@ namespace {
 constexpr int kPos = 32 * 361;
 const char *kNetName[2] = {"policy", "value"};
 
-// Milliseconds of one forward (median of 5 after 2 warm-ups).
+// Milliseconds of one forward (median of 5 after 2 warm-ups); a ragged
+// batch when `ragged` is given.
 int time_forward(lz_ctx *ctx, int net, int precision, int n,
                  const uint8_t *d_planes, float *d_raw, float *d_out,
-                 float *ms_out, std::string *msg) {
+                 float *ms_out, std::string *msg,
+                 const lz::RaggedLayout *ragged) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         *msg = "event create failed";
@@ -37,7 +39,7 @@ int time_forward(lz_ctx *ctx, int net, int precision, int n,
         if (e == hipSuccess)
             rc = lz::run_net(ctx->net[net], net, precision, n, d_planes, d_raw,
                              d_out, ctx->temperature, ctx->ws, ctx->stream,
-                             msg);
+                             msg, nullptr, ragged);
         if (rc == LZ_OK && e == hipSuccess) e = hipEventRecord(b, ctx->stream);
         if (rc == LZ_OK && e == hipSuccess) e = hipEventSynchronize(b);
         float ms = 0.0f;
@@ -57,16 +59,11 @@ int time_forward(lz_ctx *ctx, int net, int precision, int n,
     return rc;
 }
 
-}  // namespace
-
-extern "C" {
-
-int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
-                size_t report_len) {
-    if (!ctx || n < 1 || n > 65536) return LZ_ERR_ARG;
-    if (precision != LZ_PREC_BF16 && precision != LZ_PREC_FP16)
-        return LZ_ERR_ARG;
-    std::lock_guard<std::mutex> g(ctx->mu);
+// Times the candidates for n boards (all 19x19, or of sides[0..n)) and
+// keeps the fastest per net for the batch's bucket (its slot count for a
+// ragged batch). ctx->mu held.
+int autotune_locked(lz_ctx *ctx, int precision, int n, const int32_t *sides,
+                    char *report, size_t report_len) {
     if (!ctx->finalized)
         return ctx->fail(LZ_ERR_STATE, "autotune before lz_finalize");
     if (!ctx->net[0].fused_ok || !ctx->net[1].fused_ok)
@@ -75,9 +72,20 @@ int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
     std::string msg;
     int rc = lz::ensure_workspace(ctx->ws, ctx->net, n, &msg);
     if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+    size_t in_bytes = (size_t)n * kPos, out_floats = (size_t)n * 361;
+    const lz::RaggedLayout *ragged = nullptr;
+    int slots = n;
+    if (sides) {
+        rc = lz::build_ragged_layout(ctx->ragged, n, sides, ctx->stream, &msg);
+        if (rc != LZ_OK) return ctx->fail(rc, "%s", msg.c_str());
+        ragged = &ctx->ragged;
+        slots = ctx->ragged.nslots;
+        in_bytes = ctx->ragged.plane_bytes;
+        out_floats = ctx->ragged.policy_floats;
+    }
     // Inputs: a fixed pseudo-random 0/1 pattern (the kernels' time does not
     // depend on the values).
-    std::vector<uint8_t> h((size_t)n * kPos);
+    std::vector<uint8_t> h(in_bytes);
     uint32_t x = 2463534242u;
     for (auto &v : h) {
         x ^= x << 13;
@@ -90,14 +98,14 @@ int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&d_planes), h.size());
     if (e == hipSuccess)
         e = hipMalloc(reinterpret_cast<void **>(&d_raw),
-                      sizeof(float) * (size_t)n * 361);
+                      sizeof(float) * out_floats);
     if (e == hipSuccess)
         e = hipMalloc(reinterpret_cast<void **>(&d_out),
-                      sizeof(float) * (size_t)n * 361);
+                      sizeof(float) * out_floats);
     if (e == hipSuccess)
         e = hipMemcpy(d_planes, h.data(), h.size(), hipMemcpyHostToDevice);
     std::string rep;
-    const int bucket = lz::tune_bucket(n);
+    const int bucket = lz::tune_bucket(slots);
     for (int net = 0; net < 2 && e == hipSuccess && rc == LZ_OK; net++) {
         lz::Net &N = ctx->net[net];
         lz::Net::Tune best;
@@ -107,20 +115,21 @@ int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
                 N.tune[bucket] = lz::Net::Tune{wm, gm, 0.0f};
                 float ms = 0.0f;
                 rc = time_forward(ctx, net, precision, n, d_planes, d_raw,
-                                  d_out, &ms, &msg);
+                                  d_out, &ms, &msg, ragged);
                 if (rc != LZ_OK) break;
                 char line[128];
                 snprintf(line, sizeof line,
-                         "%s n=%d wm=%d grid_mult=%d: %.4f ms\n",
-                         kNetName[net], n, wm, gm, ms);
+                         "%s n=%d slots=%d wm=%d grid_mult=%d: %.4f ms\n",
+                         kNetName[net], n, slots, wm, gm, ms);
                 rep += line;
                 if (best.ms < 0.0f || ms < best.ms)
                     best = lz::Net::Tune{wm, gm, ms};
             }
         N.tune[bucket] = best;
         char line[128];
-        snprintf(line, sizeof line, "%s n=%d chosen wm=%d grid_mult=%d %.4f ms\n",
-                 kNetName[net], n, best.wm, best.grid_mult, best.ms);
+        snprintf(line, sizeof line,
+                 "%s n=%d slots=%d chosen wm=%d grid_mult=%d %.4f ms\n",
+                 kNetName[net], n, slots, best.wm, best.grid_mult, best.ms);
         rep += line;
     }
     (void)hipFree(d_planes);
@@ -134,6 +143,31 @@ int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
         report[report_len - 1] = 0;
     }
     return LZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lz_autotune(lz_ctx *ctx, int precision, int n, char *report,
+                size_t report_len) {
+    if (!ctx || n < 1 || n > 65536) return LZ_ERR_ARG;
+    if (precision != LZ_PREC_BF16 && precision != LZ_PREC_FP16)
+        return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return autotune_locked(ctx, precision, n, nullptr, report, report_len);
+}
+
+int lz_autotune_ragged(lz_ctx *ctx, int precision, int n,
+                       const int32_t *sides, char *report,
+                       size_t report_len) {
+    if (!ctx || n < 1 || n > 65536 || !sides) return LZ_ERR_ARG;
+    if (precision != LZ_PREC_BF16 && precision != LZ_PREC_FP16)
+        return LZ_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (lz::ragged_slot_capacity(sides[i]) == 0) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return autotune_locked(ctx, precision, n, sides, report, report_len);
 }
 
 int lz_autotune_save(lz_ctx *ctx, const char *path) {

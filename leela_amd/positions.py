@@ -333,6 +333,36 @@ def batch_planes(n: int, net: int = 0, seed0: int = 1) -> np.ndarray:
                       net=net)
 
 
+# The mixed-board workload: (side, boards, move cap = 0.6 x area, first
+# seed); seed ranges are disjoint.
+RAGGED_MIX = ((19, 1024, 216, 30001), (13, 512, 101, 40001),
+              (9, 512, 48, 50001))
+
+
+def ragged_mix(mix=RAGGED_MIX, pattern=(19, 13, 19, 9)):
+    """Boards of mixed sides from the generator, interleaved by `pattern`
+    (19, 13, 19, 9, ...): (sides, policy planes, value planes), the planes
+    as lists of uint8 [32][S*S]."""
+    per_side = {}
+    for side, n, mmax, seed0 in mix:
+        per_side[side] = (make_batch(n, seed0=seed0, side=side,
+                                     max_moves=mmax, net=NET_POLICY),
+                          make_batch(n, seed0=seed0, side=side,
+                                     max_moves=mmax, net=NET_VALUE))
+    total = sum(n for _, n, _, _ in mix)
+    idx = {side: 0 for side, _, _, _ in mix}
+    order = []
+    while len(order) < total:
+        for s in pattern:
+            if idx[s] < len(per_side[s][0]):
+                order.append((s, idx[s]))
+                idx[s] += 1
+    sides = [s for s, _ in order]
+    pol = [per_side[s][0][i] for s, i in order]
+    val = [per_side[s][1][i] for s, i in order]
+    return sides, pol, val
+
+
 def synthetic_planes(n: int, seed: int, side: int = 19) -> np.ndarray:
     """uint8 [n][32][side*side] stacks shaped like the encoder's output but
     drawn independently per plane (no board engine): planes 0/1/2 partition

@@ -51,6 +51,7 @@ def lib():
         L.orc_forward_f64.argtypes = [C.c_int, C.POINTER(_Layer), C.c_int,
                                       C.c_int, C.c_double, C.c_int, u8p, f64p,
                                       f64p, C.c_int]
+        L.orc_forward_f64_gemm.argtypes = L.orc_forward_f64.argtypes
         L.orc_im2col.argtypes = [C.c_int, C.c_int, C.c_int, f32p, f32p]
         L.orc_softmax_f32.argtypes = [C.c_int, f32p, f32p, C.c_float]
         L.orc_convolve_f32.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
@@ -141,17 +142,20 @@ def forward_f32(net: Net, planes: np.ndarray, head: int, S: int = 19,
 
 
 def forward_f64(net: Net, planes: np.ndarray, head: int, S: int = 19,
-                temperature: float = 0.75, threads: int = 0):
-    """FP64 direct-convolution forward. Returns (raw, out) float64."""
+                temperature: float = 0.75, threads: int = 0,
+                gemm: bool = False):
+    """FP64 forward, direct convolution (or im2col + dgemm when gemm is
+    true: the same values up to FP64 summation order, much faster with
+    OpenBLAS loaded). Returns (raw, out) float64."""
     planes = np.ascontiguousarray(planes, dtype=np.uint8)
     n = planes.shape[0]
     per = S * S if head == HEAD_POLICY else 1
     raw = np.zeros((n, per), np.float64)
     out = np.zeros((n, per), np.float64)
-    rc = lib().orc_forward_f64(len(net.layers), net._arr, S, head,
-                               temperature, n, _p(planes, C.c_uint8),
-                               _p(raw, C.c_double), _p(out, C.c_double),
-                               threads)
+    fn = lib().orc_forward_f64_gemm if gemm else lib().orc_forward_f64
+    rc = fn(len(net.layers), net._arr, S, head, temperature, n,
+            _p(planes, C.c_uint8), _p(raw, C.c_double), _p(out, C.c_double),
+            threads)
     if rc != 0:
         raise ValueError(f"orc_forward_f64 failed: {rc}")
     return raw, out

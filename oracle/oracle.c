@@ -113,11 +113,19 @@ typedef void (*sgemm32_fn)(int, int, int, int, int, int, float, const float *,
 typedef void (*sgemv32_fn)(int, int, int, int, float, const float *, int,
                            const float *, int, float, float *, int);
 typedef void (*setthreads_fn)(int);
+typedef void (*dgemm64_fn)(int, int, int, int64_t, int64_t, int64_t, double,
+                           const double *, int64_t, const double *, int64_t,
+                           double, double *, int64_t);
+typedef void (*dgemm32_fn)(int, int, int, int, int, int, double,
+                           const double *, int, const double *, int, double,
+                           double *, int);
 
 static sgemm64_fn g_sgemm64;
 static sgemv64_fn g_sgemv64;
 static sgemm32_fn g_sgemm32;
 static sgemv32_fn g_sgemv32;
+static dgemm64_fn g_dgemm64;
+static dgemm32_fn g_dgemm32;
 static char g_blas_name[256] = "oracle loop sgemm (no BLAS loaded)";
 
 enum { CblasRowMajor = 101, CblasNoTrans = 111 };
@@ -139,6 +147,10 @@ int orc_load_blas(const char *path) {
             break;
         }
     }
+    static const char *dgemm64[] = {"scipy_cblas_dgemm64_", "cblas_dgemm64_"};
+    for (int i = 0; i < 2 && !g_dgemm64; i++)
+        g_dgemm64 = (dgemm64_fn)dlsym(h, dgemm64[i]);
+    if (!g_dgemm64) g_dgemm32 = (dgemm32_fn)dlsym(h, "cblas_dgemm");
     if (!g_sgemm64) {
         g_sgemm32 = (sgemm32_fn)dlsym(h, "cblas_sgemm");
         g_sgemv32 = (sgemv32_fn)dlsym(h, "cblas_sgemv");
@@ -186,6 +198,30 @@ void orc_sgemm_rm_nn(int M, int N, int K, const float *A, int lda,
     }
 }
 
+/* C = A B in FP64, row-major, no transposes (BLAS dgemm when loaded). */
+void orc_dgemm_rm_nn(int M, int N, int K, const double *A, int lda,
+                     const double *B, int ldb, double *C, int ldc) {
+    if (g_dgemm64) {
+        g_dgemm64(CblasRowMajor, CblasNoTrans, CblasNoTrans, M, N, K, 1.0, A,
+                  lda, B, ldb, 0.0, C, ldc);
+        return;
+    }
+    if (g_dgemm32) {
+        g_dgemm32(CblasRowMajor, CblasNoTrans, CblasNoTrans, M, N, K, 1.0, A,
+                  lda, B, ldb, 0.0, C, ldc);
+        return;
+    }
+    for (int i = 0; i < M; i++) {
+        double *c = C + (size_t)i * ldc;
+        for (int j = 0; j < N; j++) c[j] = 0.0;
+        for (int k = 0; k < K; k++) {
+            const double a = A[(size_t)i * lda + k];
+            const double *b = B + (size_t)k * ldb;
+            for (int j = 0; j < N; j++) c[j] += a * b[j];
+        }
+    }
+}
+
 static void sgemv_rm_n(int M, int N, const float *A, const float *x,
                        float *y) {
     if (g_sgemv64) {
@@ -228,6 +264,26 @@ void orc_im2col(int channels, int filter, int S, const float *in,
                 }
             }
         }
+    }
+}
+
+/* The same column layout in FP64. */
+static void im2col_f64(int channels, int filter, int S, const double *in,
+                       double *col) {
+    const int pad = filter / 2;
+    for (int c = 0; c < channels; c++) {
+        const double *plane = in + (size_t)c * S * S;
+        for (int kr = 0; kr < filter; kr++)
+            for (int kc = 0; kc < filter; kc++)
+                for (int r = 0; r < S; r++) {
+                    const int ir = r - pad + kr;
+                    for (int q = 0; q < S; q++) {
+                        const int ic = q - pad + kc;
+                        *col++ = (ir >= 0 && ir < S && ic >= 0 && ic < S)
+                                     ? plane[ir * S + ic]
+                                     : 0.0;
+                    }
+                }
     }
 }
 
@@ -283,7 +339,8 @@ typedef struct fwd_job {
     double temperature;
     const uint8_t *planes;
     void *raw, *out;
-    int f64;
+    int f64;             /* 0 FP32 reference path, 1 FP64 direct, 2 FP64 GEMM */
+    double *const *wd;   /* FP64 copies of the conv weights (mode 2) */
     int status;
 } fwd_job;
 
@@ -391,7 +448,22 @@ void orc_conv_direct_f64(int filter, int cin, int cout, int S,
     free(padded);
 }
 
-static void fwd_one_f64(const fwd_job *j, int pos, double *a, double *b) {
+/* FP64 conv as im2col + dgemm + bias + ELU (mode 2): the same arithmetic
+ * as orc_conv_direct_f64 up to FP64 summation order. */
+static void conv_gemm_f64(const orc_layer *L, const double *wd, int S,
+                          const double *in, double *out, double *col) {
+    const int P = S * S, K = L->cin * L->filter * L->filter;
+    im2col_f64(L->cin, L->filter, S, in, col);
+    orc_dgemm_rm_nn(L->cout, P, K, wd, K, col, P, out, P);
+    for (int o = 0; o < L->cout; o++)
+        for (int p = 0; p < P; p++) {
+            double v = (double)L->b[o] + out[(size_t)o * P + p];
+            out[(size_t)o * P + p] = v <= 0.0 ? expm1(v) : v;
+        }
+}
+
+static void fwd_one_f64(const fwd_job *j, int pos, double *a, double *b,
+                        double *col) {
     const int S = j->S, P = S * S, cin0 = j->L[0].cin;
     const uint8_t *pl = j->planes + (size_t)pos * cin0 * P;
     for (int i = 0; i < cin0 * P; i++) a[i] = (double)pl[i];
@@ -399,8 +471,11 @@ static void fwd_one_f64(const fwd_job *j, int pos, double *a, double *b) {
     for (int l = 0; l < j->nlayers; l++) {
         const orc_layer *L = &j->L[l];
         if (L->kind == ORC_CONV) {
-            orc_conv_direct_f64(L->filter, L->cin, L->cout, S, a, L->w, L->b,
-                                b, 1);
+            if (j->f64 == 2)
+                conv_gemm_f64(L, j->wd[l], S, a, b, col);
+            else
+                orc_conv_direct_f64(L->filter, L->cin, L->cout, S, a, L->w,
+                                    L->b, b, 1);
             len = L->cout * P;
         } else {
             if (L->cin == 361 && len == P && S < 19) embed_plane_f64(a, S);
@@ -440,14 +515,19 @@ static void *fwd_worker(void *arg) {
     if (j->f64) {
         double *a = (double *)malloc(sizeof(double) * act);
         double *b = (double *)malloc(sizeof(double) * act);
-        if (!a || !b) {
+        double *col =
+            j->f64 == 2 ? (double *)malloc(sizeof(double) *
+                                           max_col(j->nlayers, j->L, j->S))
+                        : NULL;
+        if (!a || !b || (j->f64 == 2 && !col)) {
             j->status = -1;
         } else {
             for (int p = j->tid; p < j->n; p += j->nthreads)
-                fwd_one_f64(j, p, a, b);
+                fwd_one_f64(j, p, a, b, col);
         }
         free(a);
         free(b);
+        free(col);
     } else {
         float *a = (float *)malloc(sizeof(float) * act);
         float *b = (float *)malloc(sizeof(float) * act);
@@ -499,11 +579,26 @@ static int run_forward(int nlayers, const orc_layer *L, int S, int head,
     if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
     if (threads < 1) threads = 1;
     if (threads > n) threads = n > 0 ? n : 1;
+    /* Mode 2: FP64 copies of the conv weights, shared by the workers. */
+    double **wd = (double **)calloc(nlayers, sizeof(double *));
+    if (!wd) return -7;
+    for (int l = 0; f64 == 2 && l < nlayers; l++) {
+        if (L[l].kind != ORC_CONV) continue;
+        const size_t cnt = (size_t)L[l].cout * L[l].cin * L[l].filter *
+                           L[l].filter;
+        wd[l] = (double *)malloc(sizeof(double) * cnt);
+        if (!wd[l]) {
+            for (int q = 0; q < l; q++) free(wd[q]);
+            free(wd);
+            return -7;
+        }
+        for (size_t i = 0; i < cnt; i++) wd[l][i] = (double)L[l].w[i];
+    }
     pthread_t *th = (pthread_t *)calloc(threads, sizeof(pthread_t));
     fwd_job *jobs = (fwd_job *)calloc(threads, sizeof(fwd_job));
     for (int t = 0; t < threads; t++) {
         fwd_job j = {nlayers, L, S, head, n, t, threads, T, planes, raw, out,
-                     f64, 0};
+                     f64, wd, 0};
         jobs[t] = j;
         pthread_create(&th[t], NULL, fwd_worker, &jobs[t]);
     }
@@ -514,6 +609,8 @@ static int run_forward(int nlayers, const orc_layer *L, int S, int head,
     }
     free(th);
     free(jobs);
+    for (int l = 0; l < nlayers; l++) free(wd[l]);
+    free(wd);
     return status;
 }
 
@@ -529,4 +626,12 @@ int orc_forward_f64(int nlayers, const orc_layer *layers, int S, int head,
                     double *raw, double *out, int threads) {
     return run_forward(nlayers, layers, S, head, temperature, n, planes, raw,
                        out, threads, 1);
+}
+
+int orc_forward_f64_gemm(int nlayers, const orc_layer *layers, int S,
+                         int head, double temperature, int n,
+                         const uint8_t *planes, double *raw, double *out,
+                         int threads) {
+    return run_forward(nlayers, layers, S, head, temperature, n, planes, raw,
+                       out, threads, 2);
 }

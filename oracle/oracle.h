@@ -73,6 +73,16 @@ int orc_forward_f64(int nlayers, const orc_layer *layers, int S, int head,
                     double temperature, int n, const uint8_t *planes,
                     double *raw, double *out, int threads);
 
+/* The FP64 forward with each conv as im2col + dgemm (OpenBLAS when loaded;
+ * one BLAS thread per worker): the FP64 reference at batch sizes where the
+ * direct form is slow. Pinned against orc_forward_f64 by the tests. */
+int orc_forward_f64_gemm(int nlayers, const orc_layer *layers, int S,
+                         int head, double temperature, int n,
+                         const uint8_t *planes, double *raw, double *out,
+                         int threads);
+void orc_dgemm_rm_nn(int M, int N, int K, const double *A, int lda,
+                     const double *B, int ldb, double *C, int ldc);
+
 /* FP64 direct same-padded cross-correlation + bias (+ ELU when elu != 0). */
 void orc_conv_direct_f64(int filter, int cin, int cout, int S,
                          const double *in, const float *w, const float *b,

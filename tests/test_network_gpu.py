@@ -223,3 +223,25 @@ def test_netbench(be):
     finally:
         be.queue_stop()
     assert min(direct) > 0 and min(queued) > 0
+
+
+def test_first_layer_must_take_32_planes():
+    """Every input path lays out 32 planes per board, so a net whose first
+    conv takes another count is refused at lz_finalize."""
+    import leela_amd
+    from leela_amd.backend import LeelaError
+    be = leela_amd.Backend(0)
+    try:
+        rng = np.random.default_rng(0)
+        be.push_convolve(leela_amd.NET_POLICY, 5, 16, 32,
+                         rng.standard_normal((32, 16, 5, 5)) * 0.1,
+                         np.zeros(32))
+        be.push_convolve(leela_amd.NET_POLICY, 3, 32, 1,
+                         rng.standard_normal((1, 32, 3, 3)) * 0.1,
+                         np.zeros(1))
+        with pytest.raises(LeelaError) as e:
+            be.finalize()
+        assert e.value.code == leela_amd.backend.LZ_ERR_UNSUPPORTED
+        assert "32 input planes" in str(e.value)
+    finally:
+        be.close()

@@ -208,6 +208,34 @@ def test_reference_shapes_f32_vs_f64(weights_b):
         assert np.abs(o32 - o64).max() < 1e-5
 
 
+def test_f64_gemm_equals_direct(small_nets, weights_a):
+    """The im2col + dgemm FP64 forward equals the FP64 direct convolution
+    (pinned above by Python loops) up to summation order, on small nets
+    with both heads and on the reference shapes at 19x19 and 9x9 (the
+    value head's 361-input embedding included)."""
+    from leela_amd.positions import make_batch
+    oracle.load_blas()
+    pol, val = small_nets
+    rng = np.random.default_rng(3)
+    planes = (rng.random((4, 4, 49)) < 0.3).astype(np.uint8)
+    for layers, head in ((pol, 0), (val, 1)):
+        net = oracle.Net(layers)
+        a = oracle.forward_f64(net, planes, head, S=7)
+        b = oracle.forward_f64(net, planes, head, S=7, gemm=True)
+        np.testing.assert_allclose(b[0], a[0], atol=1e-12)
+        np.testing.assert_allclose(b[1], a[1], atol=1e-12)
+    pa, va = weights_a
+    for side in (19, 9):
+        for layers, head, net_id in ((pa, 0, 0), (va, 1, 1)):
+            planes = make_batch(2, seed0=77, side=side, max_moves=40,
+                                net=net_id)
+            net = oracle.Net(layers)
+            a = oracle.forward_f64(net, planes, head, S=side)
+            b = oracle.forward_f64(net, planes, head, S=side, gemm=True)
+            np.testing.assert_allclose(b[0], a[0], atol=1e-10)
+            np.testing.assert_allclose(b[1], a[1], atol=1e-10)
+
+
 def test_fp8_e4m3_rounding_matches_torch():
     """The FP8 emulation's e4m3 rounding is torch's float8_e4m3fn (OCP
     E4M3FN, round to nearest even) on every finite input range, with
@@ -224,3 +252,30 @@ def test_fp8_e4m3_rounding_matches_torch():
     ref = torch.from_numpy(x).to(torch.float8_e4m3fn).float().numpy()
     np.testing.assert_array_equal(fp8.e4m3(x).astype(np.float32), ref)
     assert fp8.e4m3(1000.0) == 448.0 and fp8.e4m3(-1e9) == -448.0
+
+
+@pytest.mark.parametrize("side", [9, 13, 19])
+def test_small_board_value_embedding_closed_form(side):
+    """The value head of a board smaller than 19x19 sees its head plane at
+    index y*19+x of the 361-input inner product (zeros elsewhere): on a net
+    whose trunk copies plane 0 to the head plane, the FP64 oracle equals
+    the closed form of that definition (a contiguous S*S embedding, or a
+    transposed one, would not)."""
+    from fp8_nets import plane_copy_value, plane_copy_value_net
+    layers = plane_copy_value_net(seed=side)
+    rng = np.random.default_rng(side)
+    planes = (rng.random((3, 32, side * side)) < 0.5).astype(np.uint8)
+    raw, win = oracle.forward_f64(oracle.Net(layers), planes,
+                                  oracle.HEAD_VALUE, S=side)
+    for i in range(3):
+        want = plane_copy_value(layers, planes[i, 0], side)
+        assert abs(raw[i, 0] - want) < 1e-12
+        assert abs(win[i, 0] - (1 + np.tanh(want)) / 2) < 1e-12
+    if side < 19:
+        # A contiguous embedding differs: the check can tell them apart.
+        e_bad = np.zeros(361)
+        e_bad[:side * side] = planes[0, 0]
+        (_, _, _, _, w1, b1), (_, _, _, _, w2, b2) = layers[-2:]
+        h = w1.astype(np.float64) @ e_bad + b1
+        h = np.where(h > 0, h, np.expm1(np.minimum(h, 0)))
+        assert abs(float(w2[0] @ h + b2[0]) - raw[0, 0]) > 1e-3

@@ -86,3 +86,32 @@ def test_ragged_rejects_fp32_and_bad_sides(be):
     with pytest.raises(LeelaError):
         be.forward_ragged(leela_amd.NET_POLICY, leela_amd.PREC_BF16,
                           [np.zeros((32, 4 * 4), np.uint8)])
+
+
+def test_small_board_value_embedding_closed_form():
+    """The GPU's value head for boards below 19x19 against the closed form
+    of its definition (head plane at y*19+x of the 361 inputs): a value net
+    whose trunk copies input plane 0 to the head plane (tests/fp8_nets.py),
+    boards of 9, 13 and 19 in one ragged batch, FP16 and BF16 (the copied
+    plane is 0/1, exact in both)."""
+    import leela_amd
+    from fp8_nets import plane_copy_value, plane_copy_value_net
+    from leela_amd import nets
+    layers = plane_copy_value_net(seed=4)
+    b = leela_amd.Backend(0)
+    try:
+        b.load_net(leela_amd.NET_POLICY, nets.weight_set("A")[0])
+        b.load_net(leela_amd.NET_VALUE, layers)
+        b.finalize()
+        rng = np.random.default_rng(12)
+        boards = [(rng.random((32, s * s)) < 0.5).astype(np.uint8)
+                  for s in (9, 13, 19, 9, 13, 9)]
+        for prec in (leela_amd.PREC_FP16, leela_amd.PREC_BF16):
+            raw, win = b.forward_ragged(leela_amd.NET_VALUE, prec, boards)
+            for i, pl in enumerate(boards):
+                s = int(round(np.sqrt(pl.shape[1])))
+                want = plane_copy_value(layers, pl[0], s)
+                assert abs(raw[i][0] - want) < 1e-4, (prec, i, s)
+                assert abs(win[i][0] - (1 + np.tanh(want)) / 2) < 1e-4
+    finally:
+        b.close()
