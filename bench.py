@@ -189,7 +189,7 @@ def ragged_leg(be, leela_amd, nets, positions, p, stream, dev, torch,
     flops = conv * pts + fc * n
     tf = flops / (dev_ms * 1e-3) / 1e12
 
-    be.queue_start(p, max_batch=2048, max_wait_us=2000)
+    be.queue_start(p, max_batch=2048, max_wait_us=20000)
     try:
         be.queue_benchmark(sides, pol, val, threads=16, rounds=1)
         st0 = be.queue_stats()

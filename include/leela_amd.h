@@ -358,6 +358,16 @@ int lz_network_scored_moves(lz_ctx *ctx, int precision,
 int lz_network_value(lz_ctx *ctx, int precision, const lz_position *pos,
                      int ensemble, float *value);
 
+/* Network::show_heatmap (Network.cpp:834-881) of get_scored_moves's
+ * result, as text into text[0..text_len): 19 lines of "%3d " per point
+ * (probability x 1000, top row first, 0 on occupied points), then with
+ * topmoves the leading moves as "%1.3f (D4)" lines until they sum to 0.85
+ * or one is below 0.01. LZ_ERR_ARG if text_len is too small (4 KB is
+ * enough). */
+int lz_network_heatmap(lz_ctx *ctx, int precision, const lz_position *pos,
+                       int ensemble, int rotation, int topmoves, char *text,
+                       size_t text_len);
+
 /* Network::benchmark (Network.cpp:147-199), the netbench command: threads
  * threads, each on a copy of pos (19x19), run ceil(2000/threads)
  * lz_network_scored_moves and then ceil(10000/threads) lz_network_value
@@ -423,6 +433,19 @@ typedef struct {
 int lz_search(lz_ctx *ctx, int precision, const lz_position *root,
               int evaluator, int playouts, int threads, int simulations,
               float c_puct, uint32_t seed, lz_search_stats *stats);
+
+/* lz_search with LZ_EVAL_NETS semantics (same expansion, priors, value and
+ * selection) but the nets evaluated by the caller's `forward`: it fills out
+ * with the softmax probabilities [n][361] (LZ_NET_POLICY) or win rates [n]
+ * (LZ_NET_VALUE) of n rotated plane stacks uint8 [n][32][361] and returns
+ * LZ_OK. For checking a search against another implementation of the same
+ * nets (the reference CPU path restated); no counterpart in the
+ * reference. */
+typedef int (*lz_host_forward_fn)(void *user, int net, int n,
+                                  const uint8_t *planes, float *out);
+int lz_search_host(lz_host_forward_fn forward, void *user,
+                   const lz_position *root, int threads, int simulations,
+                   float c_puct, uint32_t seed, lz_search_stats *stats);
 
 /* ---- Monte Carlo playouts (teacher and fallback of the surrogate) -----
  * Playout::mc_owner (Playout.cpp:174-217): the mean outcome of n playouts

@@ -73,7 +73,7 @@ struct Batch {
     State state = kFree;
     int net = 0;
     int rows = 0;      // reserved rows
-    int writers = 0;   // producers still copying planes in
+    std::atomic<int> writers{0};  // producers still copying planes in
     bool all19 = true;
     int status = LZ_OK;
     int64_t seq = 0;   // order of opening
@@ -91,7 +91,8 @@ struct Batch {
 
     void reset() {
         state = kFree;
-        rows = writers = 0;
+        rows = 0;
+        writers = 0;
         all19 = true;
         status = LZ_OK;
     }
@@ -161,12 +162,19 @@ public:
             open_[net] = nullptr;
         }
         lk.unlock();
+        // A new batch starts the dispatcher's wait timer.
+        if (row == 0) cv_.notify_one();
         std::memcpy(b->h_planes + (size_t)row * kPos, planes,
                     (size_t)32 * side * side);
-        lk.lock();
-        b->writers--;
-        lk.unlock();
-        cv_.notify_one();
+        // The dispatcher takes a closed batch once its last copy is in:
+        // the producer that finishes it wakes the dispatcher (a batch still
+        // open is taken when it fills or on the dispatcher's timer).
+        if (b->writers.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+            lk.lock();
+            const bool closed = b->state == Batch::kClosed;
+            lk.unlock();
+            if (closed) cv_.notify_one();
+        }
         return LZ_OK;
     }
 
@@ -327,7 +335,13 @@ private:
         if (b.d_in_off) hipFree(b.d_in_off);
         if (b.d_out_off) hipFree(b.d_out_off);
         free_workspace(b.ws);
-        b = Batch();
+        b.h_planes = b.d_planes = b.h_table = b.d_table = nullptr;
+        b.h_raw = b.h_out = b.d_raw = b.d_out = nullptr;
+        b.d_in_off = b.d_out_off = nullptr;
+        b.stream = nullptr;
+        b.done = nullptr;
+        b.meta.clear();
+        b.reset();
     }
 
     int capture(Batch &b, int net, int k, bool ragged, std::string *msg) {

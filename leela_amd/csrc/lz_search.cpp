@@ -162,6 +162,34 @@ private:
     int precision_;
 };
 
+// The same expansion and evaluation as NetEvaluator's synchronous path,
+// with the nets evaluated by a caller's forward (lz_search_host): a search
+// whose evaluations come from another implementation of the same nets, to
+// compare trees with.
+class ForwardEvaluator : public Evaluator {
+public:
+    explicit ForwardEvaluator(Network::ForwardFn fwd) : fwd_(std::move(fwd)) {}
+    bool expand(const go::GameState &st, bool at_root,
+                std::vector<std::pair<int, float>> &moves) override {
+        Network net(fwd_);
+        const Network::Netresult r = net.get_scored_moves(
+            st, at_root ? Network::AVERAGE_ALL : Network::DIRECT, 0);
+        if (net.last_status() != LZ_OK) return false;
+        for (const auto &sn : r)
+            if (candidate(st, sn.second)) moves.emplace_back(sn.second, sn.first);
+        return true;
+    }
+    bool value(const go::GameState &st, uint32_t, float *v) override {
+        Network net(fwd_);
+        *v = net.get_value(st, Network::DIRECT);
+        return net.last_status() == LZ_OK;
+    }
+    size_t max_children() const override { return 35; }
+
+private:
+    Network::ForwardFn fwd_;
+};
+
 // Host evaluator without a GPU: uniform priors over the candidates and the
 // mean of `playouts` light playouts (Playout::mc_owner's role in the
 // reference's search).
@@ -336,26 +364,15 @@ void walk(const Node &n, long long *vl_left, long long *bad_sums) {
 }  // namespace
 }  // namespace lz
 
-extern "C" {
+namespace lz {
+namespace {
 
-int lz_search(lz_ctx *ctx, int precision, const lz_position *root,
-              int evaluator, int playouts, int threads, int simulations,
-              float c_puct, uint32_t seed, lz_search_stats *stats) {
-    if (!root || !stats || threads < 1 || threads > 1024 || simulations < 1 ||
-        simulations > (1 << 22) || !(c_puct > 0.0f))
-        return LZ_ERR_ARG;
-    if (evaluator != LZ_EVAL_NETS && evaluator != LZ_EVAL_PLAYOUTS)
-        return LZ_ERR_ARG;
-    if (evaluator == LZ_EVAL_NETS && (!ctx || root->st.board.side() != 19))
-        return LZ_ERR_ARG;
-    if (evaluator == LZ_EVAL_PLAYOUTS && playouts < 1) return LZ_ERR_ARG;
-
-    std::unique_ptr<lz::Evaluator> ev;
-    if (evaluator == LZ_EVAL_NETS)
-        ev = std::make_unique<lz::NetEvaluator>(ctx, precision);
-    else
-        ev = std::make_unique<lz::PlayoutEvaluator>(playouts);
-    lz::Search search(*ev, c_puct, root->st);
+// Runs `simulations` simulations of `threads` threads from root with ev and
+// fills stats (queue counters from ctx when given).
+int run_search(Evaluator &ev, lz_ctx *ctx, const lz_position *root,
+               int threads, int simulations, float c_puct, uint32_t seed,
+               lz_search_stats *stats) {
+    Search search(ev, c_puct, root->st);
 
     int64_t q0[6] = {0, 0, 0, 0, 0, 0}, q1[6] = {0, 0, 0, 0, 0, 0};
     if (ctx) lz_queue_stats(ctx, q0);
@@ -415,10 +432,50 @@ int lz_search(lz_ctx *ctx, int precision, const lz_position *root,
     s.best_visits = best < 0 ? 0 : best;
     s.root_children = (int32_t)search.root.children.size();
     long long vl = 0, bad = 0;
-    lz::walk(search.root, &vl, &bad);
+    walk(search.root, &vl, &bad);
     s.virtual_loss_left = vl;
     s.visit_sum_mismatches = bad;
     return LZ_OK;
+}
+
+}  // namespace
+}  // namespace lz
+
+extern "C" {
+
+int lz_search(lz_ctx *ctx, int precision, const lz_position *root,
+              int evaluator, int playouts, int threads, int simulations,
+              float c_puct, uint32_t seed, lz_search_stats *stats) {
+    if (!root || !stats || threads < 1 || threads > 1024 || simulations < 1 ||
+        simulations > (1 << 22) || !(c_puct > 0.0f))
+        return LZ_ERR_ARG;
+    if (evaluator != LZ_EVAL_NETS && evaluator != LZ_EVAL_PLAYOUTS)
+        return LZ_ERR_ARG;
+    if (evaluator == LZ_EVAL_NETS && (!ctx || root->st.board.side() != 19))
+        return LZ_ERR_ARG;
+    if (evaluator == LZ_EVAL_PLAYOUTS && playouts < 1) return LZ_ERR_ARG;
+    std::unique_ptr<lz::Evaluator> ev;
+    if (evaluator == LZ_EVAL_NETS)
+        ev = std::make_unique<lz::NetEvaluator>(ctx, precision);
+    else
+        ev = std::make_unique<lz::PlayoutEvaluator>(playouts);
+    return lz::run_search(*ev, ctx, root, threads, simulations, c_puct, seed,
+                          stats);
+}
+
+int lz_search_host(lz_host_forward_fn forward, void *user,
+                   const lz_position *root, int threads, int simulations,
+                   float c_puct, uint32_t seed, lz_search_stats *stats) {
+    if (!forward || !root || !stats || threads < 1 || threads > 1024 ||
+        simulations < 1 || simulations > (1 << 22) || !(c_puct > 0.0f) ||
+        root->st.board.side() != 19)
+        return LZ_ERR_ARG;
+    lz::ForwardEvaluator ev(
+        [forward, user](int net, int rows, const uint8_t *in, float *out) {
+            return forward(user, net, rows, in, out);
+        });
+    return lz::run_search(ev, nullptr, root, threads, simulations, c_puct,
+                          seed, stats);
 }
 
 }  // extern "C"

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <condition_variable>
 #include <functional>
 #include <memory>
@@ -111,6 +113,7 @@ void sync_done(void *user, int status, const float *raw, const float *out) {
 }  // namespace
 
 int Network::forward_rows(int net, int rows, const uint8_t *in, float *out) {
+    if (forward_) return forward_(net, rows, in, out);
     if (queue_precision(ctx_) != precision_)
         return lz_forward(ctx_, net, precision_, rows, in, nullptr, out);
     SyncCall call;
@@ -267,6 +270,58 @@ int Network::async_scored_moves(const go::GameState &state,
     return rc;
 }
 
+namespace {
+
+// FastBoard::move_to_text (FastBoard.cpp:994-1018): column letter without
+// 'I', then the row number from 1.
+std::string move_text(const go::Board &b, int vertex) {
+    if (vertex == go::kPass) return "pass";
+    const auto xy = b.xy(vertex);
+    std::string s(1, (char)(xy.first < 8 ? 'A' + xy.first : 'A' + xy.first + 1));
+    return s + std::to_string(xy.second + 1);
+}
+
+}  // namespace
+
+std::string Network::show_heatmap(const go::GameState &state,
+                                  const Netresult &result, bool topmoves) {
+    const go::Board &b = state.board;
+    std::string text;
+    char cell[16];
+    for (int y = 18; y >= 0; y--) {
+        for (int x = 0; x < 19; x++) {
+            const int v = b.vertex(x, y);
+            float score = 0.0f;
+            for (const auto &sn : result)
+                if (sn.second == v) {
+                    score = sn.first;
+                    break;
+                }
+            std::snprintf(cell, sizeof cell, "%3d ", int(score * 1000));
+            text += cell;
+        }
+        text += "\n";
+    }
+    if (topmoves) {
+        Netresult moves = result;
+        // Descending by (probability, vertex), ties keeping their order.
+        std::stable_sort(moves.rbegin(), moves.rend());
+        float cum = 0.0f;
+        for (size_t tried = 0; cum < 0.85f && tried < moves.size(); tried++) {
+            if (moves[tried].first < 0.01f) break;
+            std::snprintf(cell, sizeof cell, "%1.3f", moves[tried].first);
+            text += std::string(cell) + " (" +
+                    move_text(b, moves[tried].second) + ")\n";
+            cum += moves[tried].first;
+        }
+    }
+    return text;
+}
+
+std::string Network::get_backend() const {
+    return ctx_ ? std::string(lz_device_name(ctx_)) : std::string("host");
+}
+
 bool Network::can_issue(int max_outstanding) const {
     return lz_queue_depth(ctx_) < max_outstanding;
 }
@@ -304,6 +359,23 @@ int lz_network_value(lz_ctx *ctx, int precision, const lz_position *pos,
     *value = net.get_value(pos->st,
                            static_cast<lz::Network::Ensemble>(ensemble));
     return net.last_status();
+}
+
+int lz_network_heatmap(lz_ctx *ctx, int precision, const lz_position *pos,
+                       int ensemble, int rotation, int topmoves, char *text,
+                       size_t text_len) {
+    if (!ctx || !pos || !text || text_len == 0 || ensemble < 0 ||
+        ensemble > 2)
+        return LZ_ERR_ARG;
+    lz::Network net(ctx, precision);
+    const auto res = net.get_scored_moves(
+        pos->st, static_cast<lz::Network::Ensemble>(ensemble), rotation);
+    if (net.last_status() != LZ_OK) return net.last_status();
+    const std::string s =
+        lz::Network::show_heatmap(pos->st, res, topmoves != 0);
+    if (s.size() + 1 > text_len) return LZ_ERR_ARG;
+    std::memcpy(text, s.c_str(), s.size() + 1);
+    return LZ_OK;
 }
 
 int lz_network_benchmark(lz_ctx *ctx, int precision, const lz_position *pos,

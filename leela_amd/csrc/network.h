@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <functional>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -19,6 +20,11 @@ public:
     using scored_node = std::pair<float, int>;  // (probability, vertex)
     using Netresult = std::vector<scored_node>;
     using ScoringCallback = std::function<void(const Netresult &)>;
+    // A forward of `rows` rotated plane stacks into softmax probabilities
+    // (policy, 361 per row) or win rates (value, one per row); LZ_OK or a
+    // status. Lets the facade run on another evaluator of the same nets.
+    using ForwardFn = std::function<int(int net, int rows,
+                                        const uint8_t *in, float *out)>;
 
     static constexpr int POLICY_CHANNELS = 32;
     static constexpr int VALUE_CHANNELS = 32;
@@ -27,6 +33,10 @@ public:
 
     // ctx must be finalized; precision is an LZ_PREC_* form.
     Network(lz_ctx *ctx, int precision) : ctx_(ctx), precision_(precision) {}
+    // The facade over a caller's forward (no context: the synchronous
+    // calls only).
+    explicit Network(ForwardFn forward)
+        : ctx_(nullptr), precision_(-1), forward_(std::move(forward)) {}
 
     // Network::get_scored_moves (Network.cpp:624-674): probabilities of the
     // empty points in row-major order, losing-ladder escapes set to 0, not
@@ -60,6 +70,18 @@ public:
 
     int last_status() const { return status_; }
 
+    // Network::show_heatmap (Network.cpp:834-881), returned as text: the
+    // 19 rows of int(probability x 1000) as "%3d " per point, top row
+    // (y = 18) first, empty for occupied points; with topmoves, then the
+    // moves in descending order as "%1.3f (D4)" until they sum to 0.85 or
+    // one falls below 0.01.
+    static std::string show_heatmap(const go::GameState &state,
+                                    const Netresult &result, bool topmoves);
+
+    // Network::get_backend (Network.cpp:1534-1555): the device the nets
+    // run on, as the OpenCL build reports its device name.
+    std::string get_backend() const;
+
     // Network::benchmark (Network.cpp:147-199): `threads` threads, each on
     // its own copy of the state, run ceil(2000 / threads) policy
     // evaluations and then ceil(10000 / threads) value evaluations, all
@@ -77,6 +99,7 @@ private:
 
     lz_ctx *ctx_;
     int precision_;
+    ForwardFn forward_;
     int status_ = LZ_OK;
 };
 

@@ -26,6 +26,8 @@ def _lib():
                                        C.POINTER(C.c_float)]
         L.lz_network_async_scored_moves.argtypes = [vp, vp, C.c_int,
                                                     MOVES_FN, vp]
+        L.lz_network_heatmap.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int,
+                                         C.c_int, C.c_char_p, C.c_size_t]
         L.lz_network_benchmark.argtypes = [vp, C.c_int, vp, C.c_int,
                                            C.POINTER(C.c_float),
                                            C.POINTER(C.c_float)]
@@ -61,6 +63,17 @@ class Network:
         if rc != LZ_OK:
             raise LeelaError(rc, "lz_network_value")
         return float(v.value)
+
+    def show_heatmap(self, pos, ensemble: int = DIRECT, rotation: int = 0,
+                     topmoves: bool = True) -> str:
+        """Network::show_heatmap's text for get_scored_moves(pos)."""
+        buf = C.create_string_buffer(8192)
+        rc = _lib().lz_network_heatmap(self._be._ctx, self.precision, pos._p,
+                                       ensemble, rotation, int(topmoves), buf,
+                                       len(buf))
+        if rc != LZ_OK:
+            raise LeelaError(rc, "lz_network_heatmap")
+        return buf.value.decode()
 
     def benchmark(self, pos, threads: int):
         """Network::benchmark (netbench): (policy p/s, value p/s)."""

@@ -10,6 +10,11 @@ from .backend import LZ_OK, LeelaError, load_library
 
 EVAL_NETS, EVAL_PLAYOUTS = 0, 1
 
+# int (*lz_host_forward_fn)(void *user, int net, int n,
+#                           const uint8_t *planes, float *out)
+HOST_FORWARD_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int,
+                              C.POINTER(C.c_uint8), C.POINTER(C.c_float))
+
 
 class SearchStats(C.Structure):
     """lz_search_stats (include/leela_amd.h)."""
@@ -42,6 +47,9 @@ def _lib():
         L.lz_search.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int,
                                 C.c_int, C.c_float, C.c_uint32,
                                 C.POINTER(SearchStats)]
+        L.lz_search_host.argtypes = [HOST_FORWARD_FN, vp, vp, C.c_int,
+                                     C.c_int, C.c_float, C.c_uint32,
+                                     C.POINTER(SearchStats)]
         L._search_bound = True
     return L
 
@@ -67,3 +75,27 @@ def search(pos, threads: int, simulations: int, *, backend=None,
     d["mean_batch"] = (d["queue_positions"] / d["queue_batches"]
                        if d["queue_batches"] else 0.0)
     return d
+
+
+def search_host(pos, forward, threads: int, simulations: int, *,
+                c_puct: float = 1.5, seed: int = 1) -> dict:
+    """lz_search_host: the net-evaluated search with the nets computed by
+    `forward(net, planes) -> out` (planes uint8 [n][32][361]; out softmax
+    probabilities [n][361] for net 0, win rates [n] for net 1)."""
+    import numpy as np
+
+    def cb(user, net, n, planes, out):
+        try:
+            pl = np.ctypeslib.as_array(planes, (n, 32, 361)).copy()
+            res = np.asarray(forward(net, pl), np.float32).reshape(-1)
+            np.ctypeslib.as_array(out, (res.size,))[:] = res
+            return LZ_OK
+        except Exception:  # reported to the search as a failed forward
+            return -2
+    fn = HOST_FORWARD_FN(cb)
+    st = SearchStats()
+    rc = _lib().lz_search_host(fn, None, pos._p, threads, simulations,
+                               c_puct, seed & 0xFFFFFFFF, C.byref(st))
+    if rc != LZ_OK:
+        raise LeelaError(rc, "lz_search_host")
+    return st.as_dict()

@@ -245,3 +245,37 @@ def test_first_layer_must_take_32_planes():
         assert "32 input planes" in str(e.value)
     finally:
         be.close()
+
+
+def test_show_heatmap_text(be):
+    """Network::show_heatmap's layout (Network.cpp:834-881): 19 rows of
+    int(p x 1000) as "%3d ", top row first, 0 on occupied points; then the
+    top moves in descending order, as "%1.3f (D4)", until 0.85 is covered
+    or a move is below 0.01."""
+    import leela_amd
+    from leela_amd.network import Network
+    pos = Position.random(41)
+    net = Network(be, leela_amd.PREC_FP32)
+    probs, verts = net.get_scored_moves(pos)
+    text = net.show_heatmap(pos)
+    lines = text.splitlines()
+    grid = [[int(t) for t in ln.split()] for ln in lines[:19]]
+    assert all(len(row) == 19 for row in grid)
+    want = np.zeros((19, 19), int)
+    for p, v in zip(probs, verts):
+        x, y = v % 21 - 1, v // 21 - 1
+        want[18 - y, x] = int(np.float32(p) * np.float32(1000))
+    assert np.array_equal(np.array(grid), want)
+    sq = pos.squares().reshape(19, 19)
+    assert all(want[18 - y, x] == 0 for y in range(19) for x in range(19)
+               if sq[y, x] != 2)
+    top = lines[19:]
+    order = sorted(zip(probs, verts), reverse=True)
+    cum = 0.0
+    for i, ln in enumerate(top):
+        p, v = order[i]
+        x, y = v % 21 - 1, v // 21 - 1
+        col = chr(ord("A") + x + (1 if x >= 8 else 0))
+        assert ln == f"{p:1.3f} ({col}{y + 1})"
+        cum += p
+    assert len(top) >= 1

@@ -75,3 +75,38 @@ def test_search_rejects_bad_arguments():
         search.search(p, 0, 10, evaluator=search.EVAL_PLAYOUTS)
     with pytest.raises(Exception):
         search.search(p, 1, 10, evaluator=search.EVAL_NETS)  # no context
+
+
+def _oracle_forward(weights):
+    import oracle
+    oracle.load_blas()
+    nets_ = (oracle.Net(weights[0]), oracle.Net(weights[1]))
+
+    def forward(net, planes):
+        head = oracle.HEAD_POLICY if net == 0 else oracle.HEAD_VALUE
+        _, out = oracle.forward_f32(nets_[net], planes, head, threads=1)
+        return out
+    return forward
+
+
+def test_search_host_with_the_reference_cpu_path(weights_a):
+    """lz_search_host with the oracle's restatement of the reference CPU
+    path as the nets: a consistent tree, the same tree on a second run (one
+    thread), and the AVERAGE_ALL root expansion passing eight rows."""
+    seen = []
+    fwd = _oracle_forward(weights_a)
+
+    def forward(net, planes):
+        seen.append((net, planes.shape[0]))
+        return fwd(net, planes)
+    p = positions.Position.random(21)
+    a = search.search_host(p, forward, 1, 24, seed=5)
+    b = search.search_host(p, fwd, 1, 24, seed=5)
+    assert a["simulations"] == 24 and a["root_visits"] == 24
+    assert a["virtual_loss_left"] == 0 and a["visit_sum_mismatches"] == 0
+    assert a["policy_evals"] == a["value_evals"]
+    assert seen[0] == (0, 8)
+    assert all(n == 1 for _, n in seen[1:])
+    for k in ("nodes", "best_move", "best_visits", "root_children",
+              "root_value", "policy_evals"):
+        assert a[k] == b[k], k

@@ -305,3 +305,35 @@ def test_training_beats_playouts_at_equal_cost():
     assert mae_sur < 0.8 * mae_const
     assert mae_sur < mae_16
     assert t_sur < t_po
+
+
+def test_config4_at_full_scale(capsys):
+    """The surrogate workload at its stated scale, through the training CLI:
+    200,000 generator positions (seeds 10000.., move counts 0..250) labelled
+    with the mean of 256 GPU playouts each, Adam at lr 1e-3 with batch 512
+    (8 epochs), inference at batch 4096, and the error judged as MAE against
+    a fresh 256-playout mean on 4,096 held-out positions. The surrogate
+    must beat a 64-playout estimate and the constant predictor, and answer
+    at batch 4096 faster than the host threads produce 256-playout
+    estimates."""
+    import json
+    from leela_amd import train_surrogate
+    print("config 4: 200000 x 256 GPU playouts, 8 epochs of batch 512",
+          flush=True)
+    rc = train_surrogate.main(["--n", "200000", "--seed0", "10000",
+                               "--playouts", "256", "--epochs", "8",
+                               "--batch", "512", "--lr", "1e-3",
+                               "--eval-n", "4096", "--eval-playouts", "256",
+                               "--infer-batch", "4096", "--log-every",
+                               "1000"])
+    assert rc == 0
+    out = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    with capsys.disabled():
+        print(f"config 4: MAE surrogate {out['mae_surrogate']:.4f}, "
+              f"64 playouts {out['mae_64_playouts']:.4f}, constant "
+              f"{out['mae_constant']:.4f}; data {out['data_seconds']} s, "
+              f"training {out['train_seconds']} s, inference "
+              f"{out['infer_positions_per_s']:.0f} positions/s at 4096")
+    assert out["mae_surrogate"] < out["mae_64_playouts"]
+    assert out["mae_surrogate"] < 0.8 * out["mae_constant"]
+    assert out["infer_positions_per_s"] > out["playout_estimates_per_s"]
