@@ -460,6 +460,42 @@ int lz_search_host(lz_host_forward_fn forward, void *user,
 int lz_playout_winrate(const lz_position *pos, int n, uint32_t seed,
                        float *winrate);
 
+/* The reference's playout policy: FastState::play_random_move
+ * (FastState.cpp:168-255) — captures of strings left in atari, saving
+ * moves for the strings the last move put in atari (escape or capture),
+ * semeai moves, nakade, and the 3x3 patterns around the last move, each
+ * scored by the product of its feature weights and its pattern's weight
+ * (Matcher / PolicyWeights, Matcher.cpp:39-112); when no candidate scores
+ * 32 (cfg_bound), four random moves from the empty list and a pass join
+ * them; one move is drawn in proportion to the scores. Its tables are read
+ * from data/mc_policy.bin next to the library (or $LZ_MC_POLICY, or the
+ * file lz_mc_policy_load names), written by
+ * leela_amd/data/make_mc_policy.py from the reference's tables. */
+#define LZ_PLAYOUT_LIGHT 0   /* uniform over legal non-eye points */
+#define LZ_PLAYOUT_PATTERN 1 /* the reference's pattern policy */
+
+/* Loads the pattern policy's tables from `path` (err receives a message on
+ * failure). Optional: the first pattern playout loads the default file. */
+int lz_mc_policy_load(const char *path, char *err, size_t err_len);
+
+/* lz_playout_winrate with the chosen playout policy (LZ_ERR_STATE when the
+ * pattern tables cannot be loaded). */
+int lz_playout_winrate_policy(const lz_position *pos, int policy, int n,
+                              uint32_t seed, float *winrate);
+
+/* The n playouts behind lz_playout_winrate_policy, one by one: scores[j] is
+ * playout j's FastBoard::final_mc_score (black's view, komi included), the
+ * playout seeded as the win rate's playout j (Playout::run with
+ * postpassout, Playout.cpp:51-113). */
+int lz_playout_scores(const lz_position *pos, int policy, int n,
+                      uint32_t seed, float *scores);
+
+/* lz_make_value_labels with the chosen playout policy. */
+int lz_make_value_labels_policy(int policy, int n, int side, uint32_t seed0,
+                                int max_moves, int playouts,
+                                uint32_t playout_seed, int threads,
+                                uint8_t *planes, float *winrates);
+
 /* FastBoard::final_mc_score (FastBoard.cpp:420-443) with the position's
  * komi: black stones plus empty points whose four neighbours are black or
  * edge, minus the same for white, minus komi. */

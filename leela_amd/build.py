@@ -39,7 +39,7 @@ def hipcc() -> str:
 # Sources with no device code and no HIP runtime calls (the board engine,
 # encoders, playouts and their C ABI): built by the host C++ compiler.
 HOST_ONLY = {"go_board.cpp", "input_planes.cpp", "playout.cpp",
-             "lz_host_abi.cpp", "lz_enc_record.cpp"}
+             "lz_host_abi.cpp", "lz_enc_record.cpp", "mc_policy.cpp"}
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-mpopcnt", "-Wall",
               f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
@@ -77,7 +77,7 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC",
-           *map(str, objs), "-o", str(tmp), "-lpthread"]
+           *map(str, objs), "-o", str(tmp), "-lpthread", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

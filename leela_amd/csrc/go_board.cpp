@@ -587,6 +587,7 @@ void GameState::init(int side, float k) {
     prev_last = 0;
     passes = 0;
     movenum = 0;
+    last_was_capture = false;
 }
 
 // FastState::play_move(color, vertex) / play_pass (FastState.cpp:345-389).
@@ -595,11 +596,13 @@ void GameState::play(int v) {
     prev_last = last_move;
     if (v == kPass || v == kResign) {
         last_move = kPass;
+        last_was_capture = false;
         if (passes < 4) passes++;
     } else {
         bool cap = false;
         komove = board.play(board.to_move(), v, &cap);
         last_move = v;
+        last_was_capture = cap;
         passes = 0;
     }
     board.set_to_move(other(board.to_move()));

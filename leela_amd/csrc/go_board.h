@@ -146,6 +146,7 @@ struct GameState {
     int prev_last = 0;
     int passes = 0;
     int movenum = 0;
+    bool last_was_capture = false;  // the last move captured stones
 
     void init(int side, float k);
     int to_move() const { return board.to_move(); }

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -11,6 +12,7 @@
 #include "input_planes.h"
 #include "leela_amd.h"
 #include "lz_position.h"
+#include "mc_policy.h"
 #include "playout.h"
 
 namespace {
@@ -188,12 +190,76 @@ int lz_playout_winrate(const lz_position *pos, int n, uint32_t seed,
     return LZ_OK;
 }
 
+int lz_mc_policy_load(const char *path, char *err, size_t err_len) {
+    std::string msg;
+    const lz::mc::Tables *t = lz::mc::tables(path, &msg);
+    if (!t && err && err_len) {
+        std::strncpy(err, msg.c_str(), err_len - 1);
+        err[err_len - 1] = 0;
+    }
+    return t ? LZ_OK : LZ_ERR_ARG;
+}
+
+int lz_playout_winrate_policy(const lz_position *pos, int policy, int n,
+                              uint32_t seed, float *winrate) {
+    if (!pos || !winrate || n < 0) return LZ_ERR_ARG;
+    if (policy == LZ_PLAYOUT_LIGHT)
+        return lz_playout_winrate(pos, n, seed, winrate);
+    if (policy != LZ_PLAYOUT_PATTERN) return LZ_ERR_ARG;
+    std::string msg;
+    const lz::mc::Tables *t = lz::mc::tables(nullptr, &msg);
+    if (!t) return LZ_ERR_STATE;
+    *winrate = lz::mc::pattern_winrate(pos->st, n, seed, *t);
+    return LZ_OK;
+}
+
+int lz_playout_scores(const lz_position *pos, int policy, int n,
+                      uint32_t seed, float *scores) {
+    if (!pos || (n > 0 && !scores) || n < 0) return LZ_ERR_ARG;
+    const lz::mc::Tables *t = nullptr;
+    if (policy == LZ_PLAYOUT_PATTERN) {
+        std::string msg;
+        t = lz::mc::tables(nullptr, &msg);
+        if (!t) return LZ_ERR_STATE;
+    } else if (policy != LZ_PLAYOUT_LIGHT) {
+        return LZ_ERR_ARG;
+    }
+    for (int j = 0; j < n; j++) {
+        lz::Rng rng(lz::go::playout_seed(seed, (uint32_t)j));
+        rng.next();
+        rng.next();
+        if (t) {
+            scores[j] = lz::mc::pattern_playout(pos->st, rng, *t);
+        } else {
+            lz::go::GameState tmp = pos->st;
+            scores[j] = lz::go::playout(tmp, rng);
+        }
+    }
+    return LZ_OK;
+}
+
 int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
                          int playouts, uint32_t playout_seed, int threads,
                          uint8_t *planes, float *winrates) {
+    return lz_make_value_labels_policy(LZ_PLAYOUT_LIGHT, n, side, seed0,
+                                       max_moves, playouts, playout_seed,
+                                       threads, planes, winrates);
+}
+
+int lz_make_value_labels_policy(int policy, int n, int side, uint32_t seed0,
+                                int max_moves, int playouts,
+                                uint32_t playout_seed, int threads,
+                                uint8_t *planes, float *winrates) {
     if (n < 0 || side < 5 || side > lz::go::kMaxSide || max_moves < 0 ||
-        playouts < 1 || !winrates)
+        playouts < 1 || !winrates ||
+        (policy != LZ_PLAYOUT_LIGHT && policy != LZ_PLAYOUT_PATTERN))
         return LZ_ERR_ARG;
+    const lz::mc::Tables *tab = nullptr;
+    if (policy == LZ_PLAYOUT_PATTERN) {
+        std::string msg;
+        tab = lz::mc::tables(nullptr, &msg);
+        if (!tab) return LZ_ERR_STATE;
+    }
     const size_t per = (size_t)lz::kPlanes * side * side;
     if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
     threads = std::max(1, std::min(threads, std::max(n, 1)));
@@ -202,8 +268,9 @@ int lz_make_value_labels(int n, int side, uint32_t seed0, int max_moves,
         for (int i = t; i < n; i += threads) {
             lz::go::random_position(st, side, seed0 + (uint32_t)i, max_moves);
             if (planes) lz::gather_features_value(st, planes + per * i);
-            winrates[i] =
-                lz::go::mc_winrate(st, playouts, playout_seed + (uint32_t)i);
+            const uint32_t s = playout_seed + (uint32_t)i;
+            winrates[i] = tab ? lz::mc::pattern_winrate(st, playouts, s, *tab)
+                              : lz::go::mc_winrate(st, playouts, s);
         }
     };
     std::vector<std::thread> pool;

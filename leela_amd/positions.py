@@ -14,6 +14,7 @@ import numpy as np
 from .backend import LZ_OK, LeelaError, load_library
 
 NET_POLICY, NET_VALUE = 0, 1
+PLAYOUT_LIGHT, PLAYOUT_PATTERN = 0, 1
 PASS = -1
 QUERY_FIELDS = 12
 BLACK, WHITE, EMPTY = 0, 1, 2
@@ -49,6 +50,14 @@ def _lib():
         L.lz_make_value_labels.argtypes = [C.c_int, C.c_int, C.c_uint32,
                                            C.c_int, C.c_int, C.c_uint32,
                                            C.c_int, u8p, f32p]
+        L.lz_make_value_labels_policy.argtypes = [
+            C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int,
+            C.c_uint32, C.c_int, u8p, f32p]
+        L.lz_playout_winrate_policy.argtypes = [vp, C.c_int, C.c_int,
+                                                C.c_uint32, f32p]
+        L.lz_mc_policy_load.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
+        L.lz_playout_scores.argtypes = [vp, C.c_int, C.c_int, C.c_uint32,
+                                        f32p]
         L.lz_encoder_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int]
         L.lz_encoder_destroy.argtypes = [vp]
         L.lz_encoder_destroy.restype = None
@@ -131,12 +140,25 @@ class Position:
         _chk(self._L.lz_position_score(self._p, C.byref(f)), "score")
         return f.value
 
-    def playout_winrate(self, n: int, seed: int) -> float:
-        """Playout::mc_owner: mean outcome of n light playouts for the
-        side to move (tie 0.5)."""
+    def playout_scores(self, n: int, seed: int,
+                       policy: int = PLAYOUT_LIGHT) -> np.ndarray:
+        """final_mc_score (black's view) of each of the n playouts behind
+        playout_winrate(n, seed, policy)."""
+        out = np.zeros(n, np.float32)
+        _chk(self._L.lz_playout_scores(self._p, policy, n, seed,
+                                       out.ctypes.data_as(
+                                           C.POINTER(C.c_float))),
+             "playout scores")
+        return out
+
+    def playout_winrate(self, n: int, seed: int,
+                        policy: int = PLAYOUT_LIGHT) -> float:
+        """Playout::mc_owner: mean outcome of n playouts for the side to
+        move (tie 0.5), light (uniform) or with the reference's pattern
+        policy."""
         f = C.c_float()
-        _chk(self._L.lz_playout_winrate(self._p, n, seed, C.byref(f)),
-             "playout")
+        _chk(self._L.lz_playout_winrate_policy(self._p, policy, n, seed,
+                                               C.byref(f)), "playout")
         return f.value
 
 
@@ -278,17 +300,28 @@ class DeviceEncoder:
         return out
 
 
+def load_mc_policy(path: str) -> None:
+    """lz_mc_policy_load: the pattern policy's tables from `path` (a file
+    written by leela_amd/data/make_mc_policy.py); raises with the loader's
+    message when the file is missing or malformed."""
+    err = C.create_string_buffer(256)
+    rc = _lib().lz_mc_policy_load(str(path).encode(), err, len(err))
+    if rc != 0:
+        raise LeelaError(rc, f"mc policy: {err.value.decode()}")
+
+
 def value_labels(n: int, seed0: int, playouts: int, playout_seed: int,
                  side: int = 19, max_moves: int = MAX_MOVES_19,
-                 threads: int = 0, planes: bool = True):
+                 threads: int = 0, planes: bool = True,
+                 policy: int = PLAYOUT_LIGHT):
     """Surrogate training data: value planes [n][32][side*side] (or None)
     of generator positions from seeds seed0.., and their `playouts`-playout
     win rates for the side to move, playouts of position i seeded
-    playout_seed + i."""
+    playout_seed + i, with the light or the pattern playout policy."""
     pl = np.zeros((n, 32, side * side), np.uint8) if planes else None
     w = np.zeros(n, np.float32)
-    _chk(_lib().lz_make_value_labels(
-        n, side, seed0, max_moves, playouts, playout_seed, threads,
+    _chk(_lib().lz_make_value_labels_policy(
+        policy, n, side, seed0, max_moves, playouts, playout_seed, threads,
         pl.ctypes.data_as(C.POINTER(C.c_uint8)) if planes else None,
         w.ctypes.data_as(C.POINTER(C.c_float))), "value_labels")
     return pl, w

@@ -18,6 +18,10 @@ The evaluation labels held-out positions (seeds eval_seed0..) with a fresh
 surrogate, of the constant predictor and of k-playout estimates, plus the
 surrogate's inference rate at batch `--infer-batch` and the playout rate on
 `--threads` host threads. Prints one JSON line.
+
+`--policy pattern` labels with the reference's pattern playout policy
+(host playouts; every label, training and evaluation, from that teacher);
+the default light policy runs on the GPU.
 """
 from __future__ import annotations
 
@@ -52,6 +56,10 @@ def main(argv=None) -> int:
     ap.add_argument("--infer-batch", type=int, default=4096)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--log-every", type=int, default=200)
+    ap.add_argument("--policy", choices=("light", "pattern"),
+                    default="light",
+                    help="playout policy of the labels (pattern: the "
+                         "reference's, on host threads)")
     ap.add_argument("--host-labels", action="store_true",
                     help="label with host playouts (default: GPU playouts, "
                          "the same values)")
@@ -61,11 +69,17 @@ def main(argv=None) -> int:
     from . import positions
     from .surrogate import Surrogate, train
 
+    policy = (positions.PLAYOUT_PATTERN if args.policy == "pattern"
+              else positions.PLAYOUT_LIGHT)
+    host = args.host_labels or policy == positions.PLAYOUT_PATTERN
+
     def make_labels(n, seed0, playouts, playout_seed0, planes=True):
-        # GPU playouts unless asked for the host's (equal values).
-        if args.host_labels:
+        # GPU playouts unless asked for the host's (equal values); the
+        # pattern policy runs on the host.
+        if host:
             return positions.value_labels(n, seed0, playouts, playout_seed0,
-                                          threads=args.threads, planes=planes)
+                                          threads=args.threads, planes=planes,
+                                          policy=policy)
         return positions.value_labels_device(n, seed0, playouts,
                                              playout_seed0, planes=planes)
 
@@ -146,11 +160,12 @@ def main(argv=None) -> int:
     t0 = time.perf_counter()
     npo = max(args.threads, 64)
     positions.value_labels(npo, args.eval_seed0, 256, 123, threads=args.threads,
-                           planes=False)
+                           planes=False, policy=policy)
     t_po = time.perf_counter() - t0
     out.update({
         "train_positions": args.n, "label_playouts": args.playouts,
-        "labels": "host playouts" if args.host_labels else "GPU playouts",
+        "labels": "host playouts" if host else "GPU playouts",
+        "policy": args.policy,
         "data_seconds": round(t_data, 2),
         "steps": steps, "batch": args.batch, "lr": args.lr,
         "first_loss": losses[0][1] if losses else None,

@@ -52,6 +52,9 @@ def lib():
         L.ref_final_score.argtypes = [C.c_int, C.c_float, C.c_int, i32p]
         L.ref_final_score.restype = C.c_float
         L.ref_rng_flt.restype = None
+        L.ref_pattern_playouts.argtypes = [C.c_int, C.c_float, C.c_int, i32p,
+                                           C.c_int, C.POINTER(C.c_uint32),
+                                           C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -102,4 +105,16 @@ def rng_u32(seed: int, n: int, max_: int = 0) -> np.ndarray:
 def rng_flt(seed: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.float32)
     lib().ref_rng_flt(seed, n, _p(out, C.c_float))
+    return out
+
+
+def pattern_playouts(side: int, komi: float, moves, seeds) -> np.ndarray:
+    """final_mc_score of one reference pattern playout per seed from the
+    position after `moves` (rebuilt in raster order; see driver.cpp)."""
+    moves = np.ascontiguousarray(moves, np.int32)
+    seeds = np.ascontiguousarray(seeds, np.uint32)
+    out = np.zeros(len(seeds), np.float32)
+    lib().ref_pattern_playouts(side, komi, len(moves), _p(moves, C.c_int32),
+                               len(seeds), _p(seeds, C.c_uint32),
+                               _p(out, C.c_float))
     return out

@@ -8,6 +8,8 @@
 
 #include "FastBoard.h"
 #include "FastState.h"
+#include "GTP.h"
+#include "Matcher.h"
 #include "Random.h"
 #include "Zobrist.h"
 
@@ -18,6 +20,21 @@ void init_once() {
     if (!done) {
         Random rng(5489);  // Leela.cpp:279-280
         Zobrist::init_zobrist(rng);
+        done = true;
+    }
+}
+
+// The playout policy's settings at their GTP defaults
+// (GTP::setup_default_parameters, GTP.cpp:64-104), then the pattern tables
+// (Leela.cpp:284).
+void init_policy_once() {
+    static bool done = false;
+    if (!done) {
+        init_once();
+        cfg_mc_softmax = 1.0f;
+        cfg_random_loops = 4;
+        cfg_bound = 32.0f;
+        Matcher::get_Matcher();
         done = true;
     }
 }
@@ -114,6 +131,52 @@ float ref_final_score(int side, float komi, int n, const int32_t *moves) {
     st.init_game(side, komi);
     for (int i = 0; i < n; i++) st.play_move(moves[i]);
     return st.board.final_mc_score(komi);
+}
+
+// Playouts of the reference's policy (FastState::play_random_move) from the
+// position after `moves`, the loop of Playout::run with postpassout and no
+// resignation (Playout.cpp:55-80). The position is rebuilt with its stones
+// placed in raster order as game moves that capture nothing
+// (FullBoard::update_board), the state fields copied from the replay: the
+// same empty-point order as the library's playout board gets from a
+// position. Playout j seeds the thread's generator with seeds[j] and
+// discards two outputs; scores[j] = calculate_mc_score at its end.
+int ref_pattern_playouts(int side, float komi, int n_moves,
+                         const int32_t *moves, int n, const uint32_t *seeds,
+                         float *scores) {
+    init_policy_once();
+    FastState st;
+    st.init_game(side, komi);
+    for (int i = 0; i < n_moves; i++) st.play_move(moves[i]);
+    FastState base;
+    base.init_game(side, komi);
+    for (int y = 0; y < side; y++)
+        for (int x = 0; x < side; x++) {
+            const int v = st.board.get_vertex(x, y);
+            const int c = st.board.get_square(v);
+            if (c != FastBoard::BLACK && c != FastBoard::WHITE) continue;
+            bool capture = false;
+            base.board.update_board(c, v, capture);
+        }
+    base.board.set_to_move(st.board.get_to_move());
+    base.m_komove = st.m_komove;
+    base.m_lastmove = st.m_lastmove;
+    base.m_onebutlastmove = st.m_onebutlastmove;
+    base.m_last_was_capture = st.m_last_was_capture;
+    base.m_passes = st.m_passes;
+    base.m_movenum = st.m_movenum;
+    const int length = 2 * side * side;
+    for (int j = 0; j < n; j++) {
+        Random *rng = Random::get_Rng();
+        rng->seedrandom(seeds[j]);
+        rng->randuint32();
+        rng->randuint32();
+        FastState tmp = base;
+        while (tmp.get_passes() < 4 && (int)tmp.get_movenum() < length)
+            tmp.play_random_move(tmp.get_to_move());
+        scores[j] = tmp.calculate_mc_score();
+    }
+    return 0;
 }
 
 void ref_rng_u32(uint32_t seed, int n, uint32_t max, uint32_t *out) {
