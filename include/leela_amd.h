@@ -66,6 +66,31 @@ int lz_push_convolve(lz_ctx *ctx, int net, int filter, int cin, int cout,
 int lz_push_innerproduct(lz_ctx *ctx, int net, int inputs, int outputs,
                          const float *w, const float *b);
 
+/* Activation of a conv (lz_push_activation). LZ_ACT_ELU is the reference's
+ * (every conv kernel applies it: Network.cpp:382-392, OpenCL.cpp convolve
+ * kernels); ReLU and none are the other forms a Caffe net's layer list
+ * takes (the shipped model_4258 / model_4750 nets: Convolution, ReLU,
+ * BatchNorm per layer, and a last Convolution with no activation). */
+#define LZ_ACT_ELU 0
+#define LZ_ACT_RELU 1
+#define LZ_ACT_NONE 2
+
+/* Sets the activation of the last conv pushed to `net` (LZ_ACT_ELU unless
+ * changed): a Caffe activation layer applied in place to that conv's
+ * output. Nets with any conv not ELU, or with batch normalisation, run in
+ * LZ_PREC_FP32 only (the other forms return LZ_ERR_UNSUPPORTED). No
+ * OpenCL counterpart: the reference's kernels hard-code ELU. */
+int lz_push_activation(lz_ctx *ctx, int net, int kind);
+
+/* Appends batch normalisation to the last conv pushed to `net`, applied to
+ * its output after the activation: out = (x - means[c] / scale) /
+ * sqrt(variances[c] / scale + 1e-5) per channel (Caffe's BatchNorm with
+ * stored statistics; channels must equal that conv's outputs). Replaces
+ * OpenCL_Network::push_batchnorm (OpenCL.h:52-61) and its kernel
+ * (OpenCL.cpp:379-405; Network.cpp:427-448). Not on the last conv. */
+int lz_push_batchnorm(lz_ctx *ctx, int net, int channels, const float *means,
+                      const float *variances, float scale);
+
 /* Validates both layer tables and uploads the weights in the packed
  * layouts every precision uses. Must follow the last push and precede any
  * forward. The reference copies weights at push time (OpenCL.cpp:474-488);
@@ -77,7 +102,9 @@ int lz_finalize(lz_ctx *ctx);
 int lz_set_softmax_temp(lz_ctx *ctx, float temperature);
 
 /* Batched forward of `net` over n positions of 19x19, host buffers in and
- * out, synchronous. Replaces OpenCL_Network::forward (OpenCL.cpp:490-569),
+ * out, synchronous. planes is uint8 [n][C][361] with C the first conv's
+ * input channels (the reference's 32 planes; a Caffe net's own count, e.g.
+ * 22, in LZ_PREC_FP32). Replaces OpenCL_Network::forward (OpenCL.cpp:490-569),
  * which evaluates one position per call and blocks on queue.finish().
  * raw or out may be NULL when not wanted. */
 int lz_forward(lz_ctx *ctx, int net, int precision, int n,

@@ -69,6 +69,9 @@ def load_library():
                                    f32p, f32p]
     L.lz_push_innerproduct.argtypes = [vp, C.c_int, C.c_int, C.c_int, f32p,
                                        f32p]
+    L.lz_push_activation.argtypes = [vp, C.c_int, C.c_int]
+    L.lz_push_batchnorm.argtypes = [vp, C.c_int, C.c_int, f32p, f32p,
+                                    C.c_float]
     L.lz_finalize.argtypes = [vp]
     L.lz_set_softmax_temp.argtypes = [vp, C.c_float]
     L.lz_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int,
@@ -169,6 +172,22 @@ class Backend:
                                                  outputs, _ptr(w), _ptr(b)),
                     "lz_push_innerproduct")
 
+    def push_activation(self, net: int, kind: int):
+        """lz_push_activation: LZ_ACT_ELU 0, LZ_ACT_RELU 1, LZ_ACT_NONE 2
+        for the last pushed conv."""
+        self._check(self._L.lz_push_activation(self._ctx, net, kind),
+                    "lz_push_activation")
+
+    def push_batchnorm(self, net: int, means, variances, scale: float):
+        """lz_push_batchnorm on the last pushed conv (Caffe BatchNorm
+        blobs: means, variances, stored scale factor)."""
+        m, v = _f32(means), _f32(variances)
+        if m.size != v.size:
+            raise ValueError("means and variances differ in length")
+        self._check(self._L.lz_push_batchnorm(self._ctx, net, m.size, _ptr(m),
+                                              _ptr(v), float(scale)),
+                    "lz_push_batchnorm")
+
     def load_net(self, net: int, layers):
         """layers: (kind, filter, cin, cout, w, b) with kind 0=conv 1=FC."""
         for kind, filt, cin, cout, w, b in layers:
@@ -185,7 +204,8 @@ class Backend:
                     "lz_set_softmax_temp")
 
     def forward(self, net: int, precision: int, planes: np.ndarray):
-        """planes uint8 [n][32][361] -> (raw, out) float32."""
+        """planes uint8 [n][C][361] (C: the first conv's inputs, 32 for the
+        reference's nets) -> (raw, out) float32."""
         planes = np.ascontiguousarray(planes, dtype=np.uint8)
         n = planes.shape[0]
         per = 361 if net == NET_POLICY else 1

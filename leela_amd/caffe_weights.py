@@ -294,6 +294,74 @@ def summary(net: CaffeNet) -> str:
     return "\n".join(out)
 
 
+# ---- running a Caffe net on the library's kernels -------------------------
+
+ACT_ELU, ACT_RELU, ACT_NONE = 0, 1, 2  # LZ_ACT_* (include/leela_amd.h)
+_ACTS = {"ReLU": ACT_RELU, "ELU": ACT_ELU}
+
+
+def layer_table(net: CaffeNet) -> List[dict]:
+    """The net as the library's layer list, in the order Caffe runs it:
+    one dict per Convolution / InnerProduct (kind 'conv' / 'fc', filter,
+    cin, cout, w, b), with the activation layer that follows a conv in
+    place ('act': ACT_*, ACT_NONE when none follows) and the BatchNorm
+    after it ('bn': (means, variances, scale factor) or None). Data,
+    Flatten, loss and Silence layers carry nothing to run."""
+    out: List[dict] = []
+    for la in net.layers:
+        if la.type == "Convolution":
+            w = la.blobs[0]
+            if w.ndim != 4 or w.shape[2] != w.shape[3]:
+                raise ValueError(f"{la.name}: weights of shape {w.shape}")
+            b = (la.blobs[1].reshape(-1) if len(la.blobs) > 1 else
+                 np.zeros(w.shape[0], np.float32))
+            out.append(dict(kind="conv", name=la.name, filter=w.shape[2],
+                            cin=w.shape[1], cout=w.shape[0], w=w, b=b,
+                            act=ACT_NONE, bn=None))
+        elif la.type == "InnerProduct":
+            w = la.blobs[0].reshape(la.blobs[0].shape[0], -1)
+            b = (la.blobs[1].reshape(-1) if len(la.blobs) > 1 else
+                 np.zeros(w.shape[0], np.float32))
+            out.append(dict(kind="fc", name=la.name, filter=0,
+                            cin=w.shape[1], cout=w.shape[0], w=w, b=b,
+                            act=ACT_NONE, bn=None))
+        elif la.type in _ACTS:
+            if not out or out[-1]["kind"] != "conv" or out[-1]["bn"]:
+                raise ValueError(f"{la.name}: activation not after a conv")
+            out[-1]["act"] = _ACTS[la.type]
+        elif la.type == "BatchNorm":
+            if not out or out[-1]["kind"] != "conv" or out[-1]["bn"]:
+                raise ValueError(f"{la.name}: BatchNorm not after a conv")
+            if abs(la.bn_eps - 1e-5) > 1e-12:
+                raise ValueError(f"{la.name}: eps {la.bn_eps} (the reference "
+                                 "normalises with 1e-5)")
+            out[-1]["bn"] = (la.blobs[0].reshape(-1), la.blobs[1].reshape(-1),
+                             float(la.blobs[2].reshape(-1)[0]))
+        elif la.type in ("Data", "Input", "Flatten", "Softmax",
+                         "SoftmaxWithLoss", "Silence", "Slice"):
+            continue
+        else:
+            raise ValueError(f"{la.name}: layer type {la.type} is not run")
+    return out
+
+
+def push_layers(backend, which: int, table: List[dict]) -> None:
+    """Pushes a layer table (layer_table) to net `which` of an unfinalized
+    leela_amd.backend.Backend: each conv, its activation and its batch
+    normalisation, in order."""
+    for L in table:
+        if L["kind"] == "conv":
+            backend.push_convolve(which, L["filter"], L["cin"], L["cout"],
+                                  L["w"], L["b"])
+            backend.push_activation(which, L["act"])
+            if L["bn"] is not None:
+                m, v, f = L["bn"]
+                backend.push_batchnorm(which, m, v, f)
+        else:
+            backend.push_innerproduct(which, L["cin"], L["cout"], L["w"],
+                                      L["b"])
+
+
 def main(argv=None) -> int:
     import argparse
     ap = argparse.ArgumentParser(description="Read a .caffemodel and write "

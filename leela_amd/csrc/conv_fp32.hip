@@ -7,8 +7,12 @@
 // (Network.cpp:344-393, Im2Col.h:9-50). The input board is staged in LDS
 // one channel chunk at a time with a zero halo, so the im2col matrix is
 // never materialised: a B-operand element is one LDS read at
-// chunk_offset(c, ky, kx) + pixel_offset(y, x). Bias and ELU are fused
-// into the epilogue (Network.cpp:382-392).
+// chunk_offset(c, ky, kx) + pixel_offset(y, x). Channels past cin in the
+// last chunk stage as zeros (their packed weight rows are zero too), so any
+// input channel count works. Bias, the activation (ELU, Network.cpp:
+// 382-392; or ReLU / none for Caffe nets) and an optional batch
+// normalisation of the activated output (Network.cpp:427-448) are fused
+// into the epilogue.
 #include "lz_internal.h"
 
 namespace lz {
@@ -25,6 +29,12 @@ __device__ __forceinline__ float elu(float v) {
     return v > 0.0f ? v : (expf(v) - 1.0f);
 }
 
+__device__ __forceinline__ float activate(float v, int act) {
+    if (act == kActElu) return elu(v);
+    if (act == kActRelu) return v > 0.0f ? v : 0.0f;
+    return v;
+}
+
 template <int F>
 struct ConvF32Cfg {
     static constexpr int CK = (F == 5) ? 8 : 16;  // input channels per chunk
@@ -36,7 +46,7 @@ template <int F, typename InT>
 __global__ __launch_bounds__(kThreads) void conv_f32_kernel(
     const InT *__restrict__ in, const float *__restrict__ wpk,
     const float *__restrict__ bias, float *__restrict__ out, int cin,
-    int cout, int S) {
+    int cout, int S, ConvEpilogue epi) {
     using C = ConvF32Cfg<F>;
     constexpr int PAD = F / 2;
     __shared__ float s_in[C::CK * C::SP * C::SP];
@@ -74,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void conv_f32_kernel(
         for (int t = 0; t < kTilesPerWave; t++) acc[m][t] = f32x4{0, 0, 0, 0};
 
     const InT *src = in + (size_t)board * cin * P;
-    const int K = cin * F * F;
+    const int K = f32_packed_k(F, cin);  // cin rounded up to whole chunks
     const f32x4 *wsrc4 = reinterpret_cast<const f32x4 *>(
         wpk + (size_t)mb * K * kF32MBlock);
     const int kq = lane >> 4;
@@ -86,7 +96,7 @@ __global__ __launch_bounds__(kThreads) void conv_f32_kernel(
             const int c = i / plane, r = i % plane;
             const int y = r / SPs - PAD, x = r % SPs - PAD;
             float v = 0.0f;
-            if (y >= 0 && y < S && x >= 0 && x < S)
+            if (y >= 0 && y < S && x >= 0 && x < S && c0 + c < cin)
                 v = (float)src[(size_t)(c0 + c) * P + y * S + x];
             s_in[i] = v;
         }
@@ -122,10 +132,16 @@ __global__ __launch_bounds__(kThreads) void conv_f32_kernel(
             const int o = mb * kF32MBlock + m * 16 + 4 * (lane >> 4) + r;
             if (o >= cout) continue;
             const float bo = bias[o];
+            const float bm = epi.bn_mean ? epi.bn_mean[o] : 0.0f;
+            const float ba = epi.bn_mean ? epi.bn_inv_std[o] : 1.0f;
 #pragma unroll
             for (int t = 0; t < kTilesPerWave; t++) {
                 const int j = (wave * kTilesPerWave + t) * 16 + (lane & 15);
-                if (j < P) dst[(size_t)o * P + j] = elu(acc[m][t][r] + bo);
+                if (j >= P) continue;
+                float v = activate(acc[m][t][r] + bo, epi.act);
+                // Network.cpp:445: scale_stddiv * (in - mean).
+                if (epi.bn_mean) v = ba * (v - bm);
+                dst[(size_t)o * P + j] = v;
             }
         }
     }
@@ -136,11 +152,12 @@ __global__ __launch_bounds__(kThreads) void conv_f32_kernel(
 constexpr int kHeadCK = 16;
 constexpr int kHeadSP = kMaxSide + 2;
 
-// Computes the ELU'd 3x3 (cin -> 1) conv of one board into s_out[0..P).
-// Each thread owns pixels tid and tid + 256.
+// Computes the activated 3x3 (cin -> 1) conv of one board into
+// s_out[0..P). Each thread owns pixels tid and tid + 256.
 __device__ void head_conv3(const float *__restrict__ src,
                            const float *__restrict__ w, float b, int cin,
-                           int S, float *s_in, float *s_w, float *s_out) {
+                           int S, int act, float *s_in, float *s_w,
+                           float *s_out) {
     const int tid = threadIdx.x;
     const int P = S * S, SPs = S + 2, plane = SPs * SPs;
     float acc0 = 0.0f, acc1 = 0.0f;
@@ -153,12 +170,12 @@ __device__ void head_conv3(const float *__restrict__ src,
             const int c = i / plane, r = i % plane;
             const int y = r / SPs - 1, x = r % SPs - 1;
             float v = 0.0f;
-            if (y >= 0 && y < S && x >= 0 && x < S)
+            if (y >= 0 && y < S && x >= 0 && x < S && c0 + c < cin)
                 v = src[(size_t)(c0 + c) * P + y * S + x];
             s_in[i] = v;
         }
         for (int i = tid; i < kHeadCK * 9; i += kThreads)
-            s_w[i] = w[c0 * 9 + i];
+            s_w[i] = c0 + i / 9 < cin ? w[c0 * 9 + i] : 0.0f;
         __syncthreads();
         for (int c = 0; c < kHeadCK; c++) {
             const float *pl = s_in + c * plane;
@@ -172,8 +189,8 @@ __device__ void head_conv3(const float *__restrict__ src,
                 }
         }
     }
-    if (p0 < P) s_out[p0] = elu(acc0 + b);
-    if (p1 < P) s_out[p1] = elu(acc1 + b);
+    if (p0 < P) s_out[p0] = activate(acc0 + b, act);
+    if (p1 < P) s_out[p1] = activate(acc1 + b, act);
     __syncthreads();
 }
 
@@ -195,13 +212,13 @@ __device__ float block_reduce(float v, bool is_max, float *s_red) {
 __global__ __launch_bounds__(kThreads) void policy_head_f32_kernel(
     const float *__restrict__ in, const float *__restrict__ w,
     const float *__restrict__ b, float *__restrict__ raw,
-    float *__restrict__ out, int cin, int S, float temperature) {
+    float *__restrict__ out, int cin, int S, float temperature, int act) {
     __shared__ float s_in[kHeadCK * kHeadSP * kHeadSP];
     __shared__ float s_w[kHeadCK * 9];
     __shared__ float s_logit[kMaxSide * kMaxSide];
     __shared__ float s_red[kThreads / 64];
     const int board = blockIdx.x, tid = threadIdx.x, P = S * S;
-    head_conv3(in + (size_t)board * cin * P, w, b[0], cin, S, s_in, s_w,
+    head_conv3(in + (size_t)board * cin * P, w, b[0], cin, S, act, s_in, s_w,
                s_logit);
     // Network::softmax (Network.cpp:450-469): exp(x/T - max/T) / sum.
     float mx = -INFINITY;
@@ -221,16 +238,17 @@ __global__ __launch_bounds__(kThreads) void policy_head_f32_kernel(
     }
 }
 
-// Last 1-channel conv + ELU of the value net, written to h0 [n][S*S] for
-// value_fc_head.
+// Last 1-channel conv + activation of the value net, written to h0
+// [n][S*S] for value_fc_head.
 __global__ __launch_bounds__(kThreads) void value_headconv_f32_kernel(
     const float *__restrict__ in, const float *__restrict__ w,
-    const float *__restrict__ b, float *__restrict__ h0, int cin, int S) {
+    const float *__restrict__ b, float *__restrict__ h0, int cin, int S,
+    int act) {
     __shared__ float s_in[kHeadCK * kHeadSP * kHeadSP];
     __shared__ float s_w[kHeadCK * 9];
     __shared__ float s_h0[kMaxSide * kMaxSide];
     const int board = blockIdx.x, tid = threadIdx.x, P = S * S;
-    head_conv3(in + (size_t)board * cin * P, w, b[0], cin, S, s_in, s_w,
+    head_conv3(in + (size_t)board * cin * P, w, b[0], cin, S, act, s_in, s_w,
                s_h0);
     for (int p = tid; p < P; p += kThreads) h0[(size_t)board * P + p] = s_h0[p];
 }
@@ -297,51 +315,59 @@ __global__ __launch_bounds__(kThreads) void value_fc_kernel(
 template <int F, typename InT>
 hipError_t launch_conv(const void *in, const float *wpk, const float *bias,
                        float *out, int n, int cin, int cout, int S,
-                       hipStream_t stream) {
-    if (cin % ConvF32Cfg<F>::CK != 0) return hipErrorInvalidValue;
+                       const ConvEpilogue &epi, hipStream_t stream) {
+    static_assert(ConvF32Cfg<F>::CK == (F == 5 ? 8 : 16),
+                  "chunk size must match f32_packed_k");
     dim3 grid(n, (cout + kF32MBlock - 1) / kF32MBlock);
     hipLaunchKernelGGL((conv_f32_kernel<F, InT>), grid, dim3(kThreads), 0,
                        stream, static_cast<const InT *>(in), wpk, bias, out,
-                       cin, cout, S);
+                       cin, cout, S, epi);
     return hipGetLastError();
+}
+
+bool act_ok(int act) {
+    return act == kActElu || act == kActRelu || act == kActNone;
 }
 
 }  // namespace
 
 hipError_t conv_f32(int filter, bool u8_input, const void *in,
                     const float *wpk, const float *bias, float *out, int n,
-                    int cin, int cout, int S, hipStream_t stream) {
-    if (S < 1 || S > kMaxSide || n <= 0) return hipErrorInvalidValue;
+                    int cin, int cout, int S, const ConvEpilogue &epi,
+                    hipStream_t stream) {
+    if (S < 1 || S > kMaxSide || n <= 0 || cin < 1 || cout < 1 ||
+        !act_ok(epi.act) || (epi.bn_mean && !epi.bn_inv_std))
+        return hipErrorInvalidValue;
     if (filter == 3)
         return u8_input ? launch_conv<3, uint8_t>(in, wpk, bias, out, n, cin,
-                                                  cout, S, stream)
+                                                  cout, S, epi, stream)
                         : launch_conv<3, float>(in, wpk, bias, out, n, cin,
-                                                cout, S, stream);
+                                                cout, S, epi, stream);
     if (filter == 5)
         return u8_input ? launch_conv<5, uint8_t>(in, wpk, bias, out, n, cin,
-                                                  cout, S, stream)
+                                                  cout, S, epi, stream)
                         : launch_conv<5, float>(in, wpk, bias, out, n, cin,
-                                                cout, S, stream);
+                                                cout, S, epi, stream);
     return hipErrorInvalidValue;
 }
 
 hipError_t policy_head_f32(const float *in, const float *w, const float *b,
                            float *raw, float *out, int n, int cin, int S,
-                           float temperature, hipStream_t stream) {
-    if (S < 1 || S > kMaxSide || n <= 0 || cin % kHeadCK != 0)
+                           float temperature, int act, hipStream_t stream) {
+    if (S < 1 || S > kMaxSide || n <= 0 || cin < 1 || !act_ok(act))
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(policy_head_f32_kernel, dim3(n), dim3(kThreads), 0,
-                       stream, in, w, b, raw, out, cin, S, temperature);
+                       stream, in, w, b, raw, out, cin, S, temperature, act);
     return hipGetLastError();
 }
 
 hipError_t value_headconv_f32(const float *in, const float *w,
                               const float *b, float *h0, int n, int cin,
-                              int S, hipStream_t stream) {
-    if (S < 1 || S > kMaxSide || n <= 0 || cin % kHeadCK != 0)
+                              int S, int act, hipStream_t stream) {
+    if (S < 1 || S > kMaxSide || n <= 0 || cin < 1 || !act_ok(act))
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(value_headconv_f32_kernel, dim3(n), dim3(kThreads), 0,
-                       stream, in, w, b, h0, cin, S);
+                       stream, in, w, b, h0, cin, S, act);
     return hipGetLastError();
 }
 

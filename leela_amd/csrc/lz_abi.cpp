@@ -132,6 +132,67 @@ int lz_push_convolve(lz_ctx *ctx, int net, int filter, int cin, int cout,
     return LZ_OK;
 }
 
+static_assert(LZ_ACT_ELU == lz::kActElu && LZ_ACT_RELU == lz::kActRelu &&
+                  LZ_ACT_NONE == lz::kActNone,
+              "activation ids");
+
+int lz_push_activation(lz_ctx *ctx, int net, int kind) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "push after lz_finalize");
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return ctx->fail(LZ_ERR_ARG, "net must be LZ_NET_POLICY or LZ_NET_VALUE");
+    if (kind != LZ_ACT_ELU && kind != LZ_ACT_RELU && kind != LZ_ACT_NONE)
+        return ctx->fail(LZ_ERR_ARG, "bad activation %d", kind);
+    auto &layers = ctx->net[net].layers;
+    if (layers.empty() || layers.back().kind != lz::Layer::kConv)
+        return ctx->fail(LZ_ERR_STATE, "activation must follow a conv");
+    if (!layers.back().bn_mean.empty())
+        return ctx->fail(LZ_ERR_STATE,
+                         "activation after the conv's batch normalisation");
+    layers.back().act = kind;
+    return LZ_OK;
+}
+
+int lz_push_batchnorm(lz_ctx *ctx, int net, int channels, const float *means,
+                      const float *variances, float scale) {
+    if (!ctx) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "push after lz_finalize");
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE)
+        return ctx->fail(LZ_ERR_ARG, "net must be LZ_NET_POLICY or LZ_NET_VALUE");
+    auto &layers = ctx->net[net].layers;
+    if (layers.empty() || layers.back().kind != lz::Layer::kConv)
+        return ctx->fail(LZ_ERR_STATE, "batch normalisation must follow a conv");
+    lz::Layer &L = layers.back();
+    if (!L.bn_mean.empty())
+        return ctx->fail(LZ_ERR_STATE, "conv already normalised");
+    if (!means || !variances || channels != L.cout || !(scale != 0.0f) ||
+        !std::isfinite(scale))
+        return ctx->fail(LZ_ERR_ARG, "bad batch normalisation (%d channels "
+                         "for a %d-channel conv, scale %g)", channels, L.cout,
+                         (double)scale);
+    // Network.cpp:436-441 (OpenCL.cpp:397-399): mean and variance over the
+    // stored scale factor, epsilon 1e-5, 1 / sqrt in FP32.
+    const float epsilon = 1e-5f;
+    std::vector<float> m(channels), inv(channels);
+    for (int c = 0; c < channels; c++) {
+        const float mean = means[c] / scale;
+        float variance = variances[c] / scale;
+        variance += epsilon;
+        if (!(variance > 0.0f) || !std::isfinite(mean))
+            return ctx->fail(LZ_ERR_ARG, "channel %d: mean %g, variance %g",
+                             c, (double)mean, (double)variance);
+        m[c] = mean;
+        inv[c] = 1.0f / std::sqrt(variance);
+    }
+    L.bn_mean = std::move(m);
+    L.bn_inv_std = std::move(inv);
+    return LZ_OK;
+}
+
 int lz_push_innerproduct(lz_ctx *ctx, int net, int inputs, int outputs,
                          const float *w, const float *b) {
     if (!ctx) return LZ_ERR_ARG;

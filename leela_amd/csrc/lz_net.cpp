@@ -44,12 +44,14 @@ int validate_net(Net &net, int which, std::string *msg) {
     net.head = nconv - 1;
     net.side = 19;
     net.max_ch = L[0].cin;
-    // Every path that feeds the nets (lz_forward's staging, the leaf queue,
-    // the facade, the encoders) lays out the reference's 32 input planes
-    // per board (POLICY_CHANNELS = VALUE_CHANNELS = 32, Network.h:48-49).
-    if (L[0].cin != 32) {
-        *msg = "the first conv must take the 32 input planes (got " +
-               std::to_string(L[0].cin) + ")";
+    net.general = false;
+    // The reference's nets take its 32 input planes (POLICY_CHANNELS =
+    // VALUE_CHANNELS = 32, Network.h:48-49), which every path that builds
+    // planes (the leaf queue, the facade, the encoders) lays out; a Caffe
+    // net takes its own count through lz_forward / lz_forward_device.
+    if (L[0].cin > kMaxInputPlanes) {
+        *msg = "the first conv takes at most " +
+               std::to_string(kMaxInputPlanes) + " input planes";
         return LZ_ERR_UNSUPPORTED;
     }
     for (int k = 0; k < nconv; k++) {
@@ -58,16 +60,18 @@ int validate_net(Net &net, int which, std::string *msg) {
                    "match the previous layer's outputs";
             return LZ_ERR_ARG;
         }
-        const int ck = L[k].filter == 5 ? 8 : 16;
-        if (L[k].cin % ck != 0) {
-            *msg = "conv " + std::to_string(k) + ": input channels must be a "
-                   "multiple of " + std::to_string(ck);
+        if (k < nconv - 1 && L[k].cout > kMaxTrunkChannels) {
+            *msg = "trunk convs wider than " +
+                   std::to_string(kMaxTrunkChannels) +
+                   " channels are not built";
             return LZ_ERR_UNSUPPORTED;
         }
-        if (k < nconv - 1 && L[k].cout > 128) {
-            *msg = "trunk convs wider than 128 channels are not built";
+        if (!L[k].bn_mean.empty() && k == nconv - 1) {
+            *msg = "batch normalisation of the 1-channel head conv is not "
+                   "built";
             return LZ_ERR_UNSUPPORTED;
         }
+        if (L[k].act != kActElu || !L[k].bn_mean.empty()) net.general = true;
         if (L[k].cout > net.max_ch) net.max_ch = L[k].cout;
     }
     if (L[net.head].cout != 1 || L[net.head].filter != 3) {
@@ -104,14 +108,16 @@ int upload_net(Net &net, int which, std::string *msg) {
         Layer &L = net.layers[k];
         std::vector<float> packed;
         if (L.kind == Layer::kConv && k < net.head) {
-            // [cout/64][K][64]: a workgroup's 64 output channels for one K
-            // row are contiguous, so the LDS copy is a straight stream.
+            // [cout/64][Kp][64]: a workgroup's 64 output channels for one K
+            // row are contiguous, so the LDS copy is a straight stream; K
+            // rows past cin * k * k (a partial last chunk) stay zero.
             const int K = L.cin * L.filter * L.filter;
+            const int Kp = f32_packed_k(L.filter, L.cin);
             const int mbs = (L.cout + kF32MBlock - 1) / kF32MBlock;
-            packed.assign((size_t)mbs * K * kF32MBlock, 0.0f);
+            packed.assign((size_t)mbs * Kp * kF32MBlock, 0.0f);
             for (int o = 0; o < L.cout; o++)
                 for (int kk = 0; kk < K; kk++)
-                    packed[((size_t)(o / kF32MBlock) * K + kk) * kF32MBlock +
+                    packed[((size_t)(o / kF32MBlock) * Kp + kk) * kF32MBlock +
                            o % kF32MBlock] = L.w[(size_t)o * K + kk];
         } else if (L.kind == Layer::kFC && k == net.fc1) {
             packed.assign(L.w.size(), 0.0f);
@@ -123,6 +129,10 @@ int upload_net(Net &net, int which, std::string *msg) {
         }
         e = upload(&L.d_w32, packed);
         if (e == hipSuccess) e = upload(&L.d_b, L.b);
+        if (e == hipSuccess && !L.bn_mean.empty()) {
+            e = upload(&L.d_bn_mean, L.bn_mean);
+            if (e == hipSuccess) e = upload(&L.d_bn_inv_std, L.bn_inv_std);
+        }
     }
     if (e != hipSuccess) {
         *msg = std::string("weight upload: ") + hipGetErrorString(e);
@@ -133,7 +143,8 @@ int upload_net(Net &net, int which, std::string *msg) {
     // 128->1; value 5x5 32->64, 10x 64->64, 64->1, FC 361->H, FC H->1.
     const auto &L = net.layers;
     const int nconv = net.head + 1;
-    bool ok = nconv <= kMaxFusedConvs && L[0].filter == 5 && L[0].cin == 32;
+    bool ok = !net.general && nconv <= kMaxFusedConvs && L[0].filter == 5 &&
+              L[0].cin == 32;
     const int c = nconv > 1 ? L[1].cout : 0;
     for (int k = 1; ok && k < net.head; k++)
         ok = L[k].filter == 3 && L[k].cout == c && (k == 1 || L[k].cin == c);
@@ -303,6 +314,9 @@ void free_net(Net &net) {
         dfree(L.d_wf16);
         dfree(L.d_wf8);
         dfree(L.d_ws8);
+        dfree(L.d_bn_mean);
+        dfree(L.d_bn_inv_std);
+        L.d_bn_mean = L.d_bn_inv_std = nullptr;
         L.d_w32 = L.d_b = nullptr;
         L.d_wbf16 = L.d_wf16 = nullptr;
         L.d_wf8 = nullptr;
@@ -369,8 +383,12 @@ static int run_fp32(const Net &net, int which, int n, const uint8_t *planes,
     for (int k = 0; k < net.n_trunk && e == hipSuccess; k++) {
         const Layer &L = net.layers[k];
         float *y = ws.act[k & 1];
+        ConvEpilogue epi;
+        epi.act = L.act;
+        epi.bn_mean = L.d_bn_mean;
+        epi.bn_inv_std = L.d_bn_inv_std;
         e = conv_f32(L.filter, u8, x, L.d_w32, L.d_b, y, n, L.cin, L.cout, S,
-                     s);
+                     epi, s);
         if (e == hipSuccess && timer) e = timer->mark(s);
         x = y;
         u8 = false;
@@ -379,12 +397,12 @@ static int run_fp32(const Net &net, int which, int n, const uint8_t *planes,
     if (e == hipSuccess) {
         if (which == LZ_NET_POLICY) {
             e = policy_head_f32(static_cast<const float *>(x), H.d_w32, H.d_b,
-                                raw, out, n, H.cin, S, temperature, s);
+                                raw, out, n, H.cin, S, temperature, H.act, s);
         } else {
             const Layer &F1 = net.layers[net.fc1];
             const Layer &F2 = net.layers[net.fc2];
             e = value_headconv_f32(static_cast<const float *>(x), H.d_w32,
-                                   H.d_b, ws.h0, n, H.cin, S, s);
+                                   H.d_b, ws.h0, n, H.cin, S, H.act, s);
             if (e == hipSuccess) e = value_fc_head(ws.h0, F1.d_w32, F1.d_b,
                                                    F2.d_w32, F2.d_b, raw, out,
                                                    n, S * S, F1.cout, s);

@@ -20,6 +20,13 @@ struct Layer {
     int cout = 0;
     std::vector<float> w;  // conv OIHW, FC [out][in]
     std::vector<float> b;
+    int act = 0;           // conv activation (kActElu / kActRelu / kActNone)
+    // Batch normalisation of the activated conv output (empty: none):
+    // mean[c] / scale and 1 / sqrt(variance[c] / scale + 1e-5), formed in
+    // FP32 as Network.cpp:438-441 forms them.
+    std::vector<float> bn_mean, bn_inv_std;
+    float *d_bn_mean = nullptr;
+    float *d_bn_inv_std = nullptr;
     // Device copies.
     float *d_w32 = nullptr;  // FP32: trunk packed [cout/64][K][64]; head conv
                              // OIHW; FC1 transposed [in][out]; FC2 as is
@@ -33,9 +40,11 @@ struct Layer {
 // A net as the reference pushes it (Network.cpp:206-233): a chain of convs
 // each followed by ELU whose last conv has one output channel, then either
 // nothing (policy: softmax head) or two inner products (value: FC+ELU, FC,
-// then (1 + tanh) / 2).
+// then (1 + tanh) / 2). A Caffe net's convs may instead take ReLU or no
+// activation and a batch normalisation after it (`general`: FP32 only).
 struct Net {
     std::vector<Layer> layers;
+    bool general = false;   // an activation other than ELU, or a BN layer
     int side = 19;
     int n_trunk = 0;    // convs before the 1-channel head conv
     int head = -1;      // index of the 1-channel conv

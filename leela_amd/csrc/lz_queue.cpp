@@ -579,6 +579,11 @@ int lz_queue_start(lz_ctx *ctx, int precision, int max_batch,
     if (precision < LZ_PREC_FP32 || precision > LZ_PREC_FP16 ||
         max_batch < 1 || max_batch > 2048 || max_wait_us < 0)
         return ctx->fail(LZ_ERR_ARG, "bad queue parameters");
+    for (int k = 0; k < 2; k++)
+        if (ctx->net[k].layers[0].cin != 32)
+            return ctx->fail(LZ_ERR_UNSUPPORTED,
+                             "the leaf queue lays out the reference's 32 "
+                             "input planes per position");
     hipSetDevice(ctx->device);
     auto q = std::make_unique<lz::LeafQueue>(ctx, precision, max_batch,
                                              max_wait_us);

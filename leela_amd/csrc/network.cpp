@@ -114,6 +114,9 @@ void sync_done(void *user, int status, const float *raw, const float *out) {
 
 int Network::forward_rows(int net, int rows, const uint8_t *in, float *out) {
     if (forward_) return forward_(net, rows, in, out);
+    // The facade encodes the reference's 32 planes per position.
+    if (ctx_->net[net].layers.empty() || ctx_->net[net].layers[0].cin != 32)
+        return LZ_ERR_UNSUPPORTED;
     if (queue_precision(ctx_) != precision_)
         return lz_forward(ctx_, net, precision_, rows, in, nullptr, out);
     SyncCall call;

@@ -24,7 +24,12 @@ HEAD_POLICY, HEAD_VALUE = 0, 1
 class _Layer(C.Structure):
     _fields_ = [("kind", C.c_int), ("filter", C.c_int), ("cin", C.c_int),
                 ("cout", C.c_int), ("w", C.POINTER(C.c_float)),
-                ("b", C.POINTER(C.c_float))]
+                ("b", C.POINTER(C.c_float)), ("act", C.c_int),
+                ("bn_means", C.POINTER(C.c_float)),
+                ("bn_vars", C.POINTER(C.c_float)), ("bn_scale", C.c_float)]
+
+
+ACT_ELU, ACT_RELU, ACT_NONE = 0, 1, 2
 
 
 def build() -> Path:
@@ -109,19 +114,41 @@ def load_blas() -> str:
 
 class Net:
     """Layer list in the reference's push order: (kind, filter, cin, cout,
-    w, b) with w OIHW for convs and [out][in] for FCs."""
+    w, b) with w OIHW for convs and [out][in] for FCs, each conv followed
+    by ELU; or (kind, filter, cin, cout, w, b, act, bn) with act ACT_* and
+    bn None or (means, variances, scale factor) for a Caffe net's conv."""
 
     def __init__(self, layers):
         self.layers = []
-        for kind, filt, cin, cout, w, b in layers:
+        for layer in layers:
+            kind, filt, cin, cout, w, b = layer[:6]
+            act = layer[6] if len(layer) > 6 else ACT_ELU
+            bn = layer[7] if len(layer) > 7 else None
             w = np.ascontiguousarray(w, dtype=np.float32).ravel()
             b = np.ascontiguousarray(b, dtype=np.float32).ravel()
-            self.layers.append((kind, filt, cin, cout, w, b))
+            if bn is not None:
+                bn = (np.ascontiguousarray(bn[0], dtype=np.float32).ravel(),
+                      np.ascontiguousarray(bn[1], dtype=np.float32).ravel(),
+                      float(bn[2]))
+            self.layers.append((kind, filt, cin, cout, w, b, act, bn))
         arr = (_Layer * len(self.layers))()
-        for i, (kind, filt, cin, cout, w, b) in enumerate(self.layers):
+        null = C.POINTER(C.c_float)()
+        for i, (kind, filt, cin, cout, w, b, act, bn) in enumerate(
+                self.layers):
             arr[i] = _Layer(kind, filt, cin, cout, _p(w, C.c_float),
-                            _p(b, C.c_float))
+                            _p(b, C.c_float), act,
+                            _p(bn[0], C.c_float) if bn else null,
+                            _p(bn[1], C.c_float) if bn else null,
+                            bn[2] if bn else 0.0)
         self._arr = arr
+
+    @classmethod
+    def from_table(cls, table):
+        """From a layer table of leela_amd.caffe_weights.layer_table (or
+        the same dicts read back from a fixture)."""
+        return cls([(0 if L["kind"] == "conv" else 1, L["filter"], L["cin"],
+                     L["cout"], L["w"], L["b"], L["act"], L["bn"])
+                    for L in table])
 
 
 def forward_f32(net: Net, planes: np.ndarray, head: int, S: int = 19,

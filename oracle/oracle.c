@@ -291,17 +291,63 @@ static inline float elu_f32(float v) {
     return v > 0.0f ? v : (expf(v) - 1.0f);
 }
 
-/* convolve<filter, channels, outputs> (Network.cpp:344-393). */
-void orc_convolve_f32(int filter, int cin, int cout, int S, const float *in,
-                      const float *w, const float *b, float *out,
-                      float *col) {
+static inline float act_f32(float v, int act) {
+    if (act == ORC_ACT_ELU) return elu_f32(v);
+    /* Caffe's ReLU: max(x, 0) (+ 0 * min(x, 0) for slope 0). */
+    if (act == ORC_ACT_RELU) return v > 0.0f ? v : 0.0f;
+    return v;
+}
+
+static inline double act_f64(double v, int act) {
+    if (act == ORC_ACT_ELU) return v <= 0.0 ? expm1(v) : v;
+    if (act == ORC_ACT_RELU) return v > 0.0 ? v : 0.0;
+    return v;
+}
+
+/* convolve<filter, channels, outputs> (Network.cpp:344-393), with the
+ * layer's activation in place of the hard-coded ELU. */
+static void convolve_act_f32(int filter, int cin, int cout, int S,
+                             const float *in, const float *w, const float *b,
+                             float *out, float *col, int act) {
     const int P = S * S;
     const int K = cin * filter * filter;
     orc_im2col(cin, filter, S, in, col);
     orc_sgemm_rm_nn(cout, P, K, w, K, col, P, out, P);
     for (int o = 0; o < cout; o++)
         for (int p = 0; p < P; p++)
-            out[o * P + p] = elu_f32(b[o] + out[o * P + p]);
+            out[o * P + p] = act_f32(b[o] + out[o * P + p], act);
+}
+
+void orc_convolve_f32(int filter, int cin, int cout, int S, const float *in,
+                      const float *w, const float *b, float *out,
+                      float *col) {
+    convolve_act_f32(filter, cin, cout, S, in, w, b, out, col, ORC_ACT_ELU);
+}
+
+/* batchnorm<channels, spatial_size> (Network.cpp:427-448), in place. */
+static void batchnorm_f32(const orc_layer *L, int P, float *x) {
+    const float epsilon = 1e-5f;
+    for (int c = 0; c < L->cout; c++) {
+        const float mean = L->bn_means[c] / L->bn_scale;
+        float variance = L->bn_vars[c] / L->bn_scale;
+        variance += epsilon;
+        const float scale_stddiv = 1.0f / sqrtf(variance);
+        for (int p = 0; p < P; p++)
+            x[c * P + p] = scale_stddiv * (x[c * P + p] - mean);
+    }
+}
+
+/* The same in FP64 (epsilon the FP32 constant's value). */
+static void batchnorm_f64(const orc_layer *L, int P, double *x) {
+    const double epsilon = (double)1e-5f;
+    for (int c = 0; c < L->cout; c++) {
+        const double mean = (double)L->bn_means[c] / L->bn_scale;
+        const double variance =
+            (double)L->bn_vars[c] / L->bn_scale + epsilon;
+        const double inv = 1.0 / sqrt(variance);
+        for (int p = 0; p < P; p++)
+            x[(size_t)c * P + p] = inv * (x[(size_t)c * P + p] - mean);
+    }
 }
 
 /* innerproduct<inputs, outputs> (Network.cpp:395-423). */
@@ -394,8 +440,9 @@ static void fwd_one_f32(const fwd_job *j, int pos, float *a, float *b,
     for (int l = 0; l < j->nlayers; l++) {
         const orc_layer *L = &j->L[l];
         if (L->kind == ORC_CONV) {
-            orc_convolve_f32(L->filter, L->cin, L->cout, S, a, L->w, L->b, b,
-                             col);
+            convolve_act_f32(L->filter, L->cin, L->cout, S, a, L->w, L->b, b,
+                             col, L->act);
+            if (L->bn_means) batchnorm_f32(L, P, b);
             len = L->cout * P;
         } else {
             if (L->cin == 361 && len == S * S && S < 19) embed_plane_f32(a, S);
@@ -441,9 +488,11 @@ void orc_conv_direct_f64(int filter, int cin, int cout, int S,
                     }
                 }
         }
+        /* elu: 1 = ELU; otherwise 1 + an ORC_ACT_* kind. */
+        const int act = elu == 1 ? ORC_ACT_ELU
+                        : elu > 1 ? elu - 1 : ORC_ACT_NONE;
         if (elu)
-            for (int p = 0; p < P; p++)
-                if (acc[p] <= 0.0) acc[p] = expm1(acc[p]);
+            for (int p = 0; p < P; p++) acc[p] = act_f64(acc[p], act);
     }
     free(padded);
 }
@@ -458,7 +507,7 @@ static void conv_gemm_f64(const orc_layer *L, const double *wd, int S,
     for (int o = 0; o < L->cout; o++)
         for (int p = 0; p < P; p++) {
             double v = (double)L->b[o] + out[(size_t)o * P + p];
-            out[(size_t)o * P + p] = v <= 0.0 ? expm1(v) : v;
+            out[(size_t)o * P + p] = act_f64(v, L->act);
         }
 }
 
@@ -475,7 +524,8 @@ static void fwd_one_f64(const fwd_job *j, int pos, double *a, double *b,
                 conv_gemm_f64(L, j->wd[l], S, a, b, col);
             else
                 orc_conv_direct_f64(L->filter, L->cin, L->cout, S, a, L->w,
-                                    L->b, b, 1);
+                                    L->b, b, 1 + L->act);
+            if (L->bn_means) batchnorm_f64(L, P, b);
             len = L->cout * P;
         } else {
             if (L->cin == 361 && len == P && S < 19) embed_plane_f64(a, S);
@@ -555,6 +605,9 @@ static int check_layers(int nlayers, const orc_layer *L, int S, int head) {
             if (!spatial || L[i].cin * S * S != width) return -2;
             if (L[i].filter != 1 && L[i].filter != 3 && L[i].filter != 5)
                 return -3;
+            if (L[i].act < ORC_ACT_ELU || L[i].act > ORC_ACT_NONE) return -8;
+            if (L[i].bn_means && (!L[i].bn_vars || L[i].bn_scale == 0.0f))
+                return -9;
             width = L[i].cout * S * S;
         } else {
             /* A 1-channel S x S plane may feed a 361-input layer when S < 19

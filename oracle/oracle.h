@@ -25,6 +25,9 @@ extern "C" {
 
 enum { ORC_CONV = 0, ORC_FC = 1 };
 enum { ORC_HEAD_POLICY = 0, ORC_HEAD_VALUE = 1 };
+/* Conv activations: the reference's ELU (Network.cpp:382-392), and the
+ * ReLU / none of a Caffe net's layer list. */
+enum { ORC_ACT_ELU = 0, ORC_ACT_RELU = 1, ORC_ACT_NONE = 2 };
 
 typedef struct orc_layer {
     int kind;        /* ORC_CONV or ORC_FC */
@@ -33,6 +36,13 @@ typedef struct orc_layer {
     int cout;        /* output channels (conv) or outputs (FC) */
     const float *w;  /* conv: OIHW [cout][cin][k][k]; FC: [cout][cin] */
     const float *b;  /* [cout] */
+    int act;         /* conv activation (ORC_ACT_*) */
+    /* Batch normalisation of the activated conv output (Network.cpp:
+     * 427-448) when bn_means is not NULL: [cout] means and variances and
+     * the stored scale factor. */
+    const float *bn_means;
+    const float *bn_vars;
+    float bn_scale;
 } orc_layer;
 
 /* xoroshiro128+ with the reference's LCG seeding (Random.cpp:27-57). */
@@ -63,7 +73,9 @@ void orc_innerproduct_f32(int inputs, int outputs, const float *in,
 void orc_softmax_f32(int n, const float *in, float *out, float temperature);
 
 /* Whole-net forwards over n positions of S x S, planes uint8 [n][cin0][S*S].
- *   policy head: raw = n x S*S (ELU'd last conv), head = softmax(raw, T)
+ * Each conv: bias, its activation, then its batch normalisation if any.
+ *   policy head: raw = n x S*S (the last conv's output), head =
+ *   softmax(raw, T)
  *   value  head: raw = n x 1 (last FC, no ELU),   head = (1 + tanh(raw)) / 2
  * threads <= 0 means one thread per online CPU. Returns 0 on success. */
 int orc_forward_f32(int nlayers, const orc_layer *layers, int S, int head,
@@ -83,7 +95,8 @@ int orc_forward_f64_gemm(int nlayers, const orc_layer *layers, int S,
 void orc_dgemm_rm_nn(int M, int N, int K, const double *A, int lda,
                      const double *B, int ldb, double *C, int ldc);
 
-/* FP64 direct same-padded cross-correlation + bias (+ ELU when elu != 0). */
+/* FP64 direct same-padded cross-correlation + bias, then ELU when elu is
+ * 1, the activation ORC_ACT_* elu - 1 when elu > 1, none when 0. */
 void orc_conv_direct_f64(int filter, int cin, int cout, int S,
                          const double *in, const float *w, const float *b,
                          double *out, int elu);

@@ -225,23 +225,38 @@ def test_netbench(be):
     assert min(direct) > 0 and min(queued) > 0
 
 
-def test_first_layer_must_take_32_planes():
-    """Every input path lays out 32 planes per board, so a net whose first
-    conv takes another count is refused at lz_finalize."""
+def test_first_layer_plane_count():
+    """A net whose first conv takes another plane count than the
+    reference's 32 (a Caffe net's own inputs) runs in FP32 on planes
+    [n][C][361] and matches the oracle; the paths that lay out the
+    reference's 32 planes (the leaf queue) and the 16-bit forms refuse
+    it."""
     import leela_amd
+    import oracle
+    from leela_amd import nets
     from leela_amd.backend import LeelaError
+    rng = np.random.default_rng(0)
+    w1 = (rng.standard_normal((32, 16, 5, 5)) * 0.1).astype(np.float32)
+    w2 = (rng.standard_normal((1, 32, 3, 3)) * 0.1).astype(np.float32)
+    b1, b2 = np.zeros(32, np.float32), np.zeros(1, np.float32)
     be = leela_amd.Backend(0)
     try:
-        rng = np.random.default_rng(0)
-        be.push_convolve(leela_amd.NET_POLICY, 5, 16, 32,
-                         rng.standard_normal((32, 16, 5, 5)) * 0.1,
-                         np.zeros(32))
-        be.push_convolve(leela_amd.NET_POLICY, 3, 32, 1,
-                         rng.standard_normal((1, 32, 3, 3)) * 0.1,
-                         np.zeros(1))
+        be.push_convolve(leela_amd.NET_POLICY, 5, 16, 32, w1, b1)
+        be.push_convolve(leela_amd.NET_POLICY, 3, 32, 1, w2, b2)
+        be.load_net(leela_amd.NET_VALUE, nets.weight_set("A")[1])
+        be.finalize()
+        planes = (rng.random((3, 16, 361)) < 0.3).astype(np.uint8)
+        raw, prob = be.forward(leela_amd.NET_POLICY, leela_amd.PREC_FP32,
+                               planes)
+        raw64, _ = oracle.forward_f64(
+            oracle.Net([(0, 5, 16, 32, w1, b1), (0, 3, 32, 1, w2, b2)]),
+            planes, oracle.HEAD_POLICY)
+        assert np.abs(raw - raw64).max() < 1e-5
         with pytest.raises(LeelaError) as e:
-            be.finalize()
+            be.forward(leela_amd.NET_POLICY, leela_amd.PREC_FP16, planes)
         assert e.value.code == leela_amd.backend.LZ_ERR_UNSUPPORTED
+        with pytest.raises(LeelaError) as e:
+            be.queue_start(leela_amd.PREC_FP32, 16, 100)
         assert "32 input planes" in str(e.value)
     finally:
         be.close()
