@@ -38,8 +38,8 @@ AGREEMENT_BAR = 0.995
 # build they were measured on are in TRAFFIC_SOURCE. Keyed by form.
 TRAFFIC_SOURCE = "profiles/fused_net_policy_hbm_traffic_b1024.md"
 TRAFFIC_B1024 = {
-    "fp16": {"bytes": 82_946_000, "fetch_kib": 39029.2, "write_kib": 2944.0},
-    "bf16": {"bytes": 52_970_000, "fetch_kib": 24390.1, "write_kib": 2944.0},
+    "fp16": {"bytes": 75_239_424, "fetch_kib": 35266.0, "write_kib": 2944.0},
+    "bf16": {"bytes": 52_370_432, "fetch_kib": 24099.5, "write_kib": 2944.0},
     "fp32": {"bytes": 579_100_000, "fetch_kib": 190213.0,
              "write_kib": 185100.0},
 }
