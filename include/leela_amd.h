@@ -517,6 +517,22 @@ int lz_playout_winrate_policy(const lz_position *pos, int policy, int n,
 int lz_playout_scores(const lz_position *pos, int policy, int n,
                       uint32_t seed, float *scores);
 
+/* lz_playout_winrates_device with the chosen playout policy. With
+ * LZ_PLAYOUT_PATTERN each playout runs on one GPU thread, the same source
+ * as the host's (its win rates equal lz_playout_winrate_policy's for the
+ * same position and seed); a position whose candidate list outgrows the
+ * device's 2,048 entries (none of millions of generator positions' move
+ * generations needs more than ~300) gets NaN instead of a win rate. The
+ * first pattern call uploads the tables and allocates ~4.5 GB of playout
+ * states (one per thread of a persistent grid of 196,608 threads), held
+ * until lz_encoder_destroy. Pattern calls on one encoder are ordered by
+ * the library, whatever their streams. */
+int lz_playout_winrates_device_policy(lz_encoder *enc,
+                                      const lz_position *const *pos, int n,
+                                      int playouts, uint32_t seed0,
+                                      int policy, float *d_winrates,
+                                      void *stream);
+
 /* lz_make_value_labels with the chosen playout policy. */
 int lz_make_value_labels_policy(int policy, int n, int side, uint32_t seed0,
                                 int max_moves, int playouts,

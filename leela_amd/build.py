@@ -44,6 +44,11 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-mpopcnt", "-Wall",
               f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
 
+# Per-source flags: the pattern playouts' scores must round as the host's
+# (no fused multiply-adds).
+EXTRA_FLAGS = {"mc_playout.hip": ["-ffp-contract=off"]}
+
+
 def host_cxx() -> str:
     for cand in (os.environ.get("CXX"), shutil.which("g++"),
                  "/opt/rocm/llvm/bin/clang++"):
@@ -62,7 +67,8 @@ def _compile(src: Path) -> Path:
     if src.name in HOST_ONLY:
         cmd = [host_cxx(), *HOST_FLAGS, "-c", str(src), "-o", str(obj)]
     else:
-        cmd = [hipcc(), *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
+        cmd = [hipcc(), *HIP_FLAGS, *EXTRA_FLAGS.get(src.name, []), "-c",
+               str(src), "-o", str(obj)]
         if src.suffix == ".cpp":
             cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)

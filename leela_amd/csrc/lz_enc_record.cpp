@@ -23,6 +23,7 @@ void pack_record(const go::GameState &st, int64_t offset, EncRecord *r) {
     r->passes = st.passes;
     r->movenum = st.movenum;
     r->komi = st.komi;
+    r->last_was_capture = st.last_was_capture ? 1 : 0;
 }
 
 }  // namespace lz

@@ -32,7 +32,8 @@ struct alignas(64) EncRecord {
     int32_t passes;            // passes in a row (playouts stop at 4)
     int32_t movenum;           // moves played (playouts stop at 2 side^2)
     float komi;
-    uint8_t pad_[20];
+    int32_t last_was_capture;  // the last move captured stones
+    uint8_t pad_[16];
 };
 static_assert(sizeof(EncRecord) == 1408, "record layout");
 

@@ -8,6 +8,7 @@
 #include <cstdint>
 
 #include "lz_enc_record.h"
+#include "mc_core.h"
 
 namespace lz {
 
@@ -27,5 +28,21 @@ hipError_t launch_encode(const EncRecord *d_rec, int n, int symmetry,
 hipError_t launch_playouts(const EncRecord *d_rec, int npos, int nplay,
                            uint32_t seed0, uint32_t *d_half_wins,
                            float *d_winrate, hipStream_t stream);
+
+// Pattern playouts (mc_core.h's policy, the reference's) of n boards,
+// nplay each, playout j of board p seeded playout_seed(seed0 + p, j):
+// d_winrate[p] = mc_winrate for the side to move, NaN if one of its
+// playouts outgrew the device's candidate list. d_states holds nstates
+// playout states (a multiple of 64, pattern_state_bytes() each): the
+// persistent grid's threads. d_half_wins and d_overflow (n entries each)
+// are scratch.
+size_t pattern_state_bytes();
+hipError_t launch_pattern_playouts(const EncRecord *d_rec, int npos,
+                                   int nplay, uint32_t seed0,
+                                   const mc::TablesView &tables,
+                                   void *d_states, int nstates,
+                                   uint32_t *d_half_wins,
+                                   uint32_t *d_overflow, float *d_winrate,
+                                   hipStream_t stream);
 
 }  // namespace lz

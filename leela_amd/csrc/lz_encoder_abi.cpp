@@ -15,12 +15,14 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "go_board.h"
 #include "input_planes.h"
 #include "leela_amd.h"
 #include "lz_encoder.h"
 #include "lz_position.h"
+#include "mc_policy.h"
 
 struct lz_encoder {
     int device = 0;
@@ -42,6 +44,20 @@ struct lz_encoder {
     hipStream_t copy_stream = nullptr;
     int next = 0;
     uint32_t *half_wins = nullptr;  // playout outcome counters, max_batch
+    // Pattern playouts (allocated on first use): the policy's tables on the
+    // device, one playout state per thread of the persistent grid, the
+    // per-position counters of abandoned playouts, and the event after the
+    // last launch that used them (the next launch waits for it, whatever
+    // its stream).
+    int32_t *pat_g = nullptr;
+    uint16_t *pat_index[2] = {nullptr, nullptr};
+    float *pat_fw = nullptr;
+    float *pat_pw = nullptr;
+    void *pat_states = nullptr;
+    int pat_nstates = 0;
+    uint32_t *pat_overflow = nullptr;
+    hipEvent_t pat_done = nullptr;
+    bool pat_used = false;
 
     int fail(int code, const char *fmt, ...) {
         char buf[512];
@@ -112,7 +128,15 @@ void lz_encoder_destroy(lz_encoder *e) {
         if (s.host) (void)hipHostFree(s.host);
     }
     if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
-    if (e->half_wins) (void)hipFree(e->half_wins);
+    if (e->pat_done) {
+        (void)hipEventSynchronize(e->pat_done);
+        (void)hipEventDestroy(e->pat_done);
+    }
+    for (void *p : {(void *)e->half_wins, (void *)e->pat_g,
+                    (void *)e->pat_index[0], (void *)e->pat_index[1],
+                    (void *)e->pat_fw, (void *)e->pat_pw, e->pat_states,
+                    (void *)e->pat_overflow})
+        if (p) (void)hipFree(p);
     delete e;
 }
 
@@ -212,6 +236,89 @@ int playouts(lz_encoder *e, const lz_position *const *pos, int n, int nplay,
     return release_slot(e, sl, s);
 }
 
+// Threads of the pattern playouts' persistent grid: 3,072 wavefronts (12
+// per CU of the part, the kernel's occupancy), each thread with its own
+// playout state (~23 KB: 4.5 GB in all). The playouts are latency-bound
+// pointer chasing, so resident wavefronts are what sets the rate: a third
+// of this grid ran 19x19 playouts 1.4x slower.
+constexpr int kPatternThreads = 196608;
+
+template <typename T>
+hipError_t upload(T **dst, const std::vector<T> &v) {
+    hipError_t err = hipMalloc(reinterpret_cast<void **>(dst),
+                               sizeof(T) * v.size());
+    if (err == hipSuccess)
+        err = hipMemcpy(*dst, v.data(), sizeof(T) * v.size(),
+                        hipMemcpyHostToDevice);
+    return err;
+}
+
+// First use: the tables on the device, the playout states, the counters.
+int pattern_setup(lz_encoder *e) {
+    if (e->pat_states) return LZ_OK;
+    std::string msg;
+    const lz::mc::Tables *t = lz::mc::tables(nullptr, &msg);
+    if (!t) return e->fail(LZ_ERR_STATE, "pattern tables: %s", msg.c_str());
+    hipError_t err = upload(&e->pat_g, t->g);
+    if (err == hipSuccess) err = upload(&e->pat_index[0], t->index[0]);
+    if (err == hipSuccess) err = upload(&e->pat_index[1], t->index[1]);
+    if (err == hipSuccess) err = upload(&e->pat_fw, t->feature_weight);
+    if (err == hipSuccess) err = upload(&e->pat_pw, t->pattern_weight);
+    if (err == hipSuccess)
+        err = hipMalloc(reinterpret_cast<void **>(&e->pat_overflow),
+                        sizeof(uint32_t) * (size_t)e->max_batch);
+    if (err == hipSuccess)
+        err = hipEventCreateWithFlags(&e->pat_done, hipEventDisableTiming);
+    if (err == hipSuccess)
+        err = hipMalloc(&e->pat_states,
+                        lz::pattern_state_bytes() * (size_t)kPatternThreads);
+    if (err != hipSuccess) {
+        if (e->pat_states) (void)hipFree(e->pat_states);
+        e->pat_states = nullptr;
+        return e->fail(LZ_ERR_NOMEM, "pattern playout setup: %s",
+                       hipGetErrorString(err));
+    }
+    e->pat_nstates = kPatternThreads;
+    return LZ_OK;
+}
+
+int pattern_playouts(lz_encoder *e, const lz_position *const *pos, int n,
+                     int nplay, uint32_t seed0, float *d_winrate,
+                     void *stream) {
+    if (n < 0 || n > e->max_batch)
+        return e->fail(LZ_ERR_ARG, "batch %d outside 0..%d", n, e->max_batch);
+    if (nplay < 0) return e->fail(LZ_ERR_ARG, "playouts %d", nplay);
+    if (n == 0) return LZ_OK;
+    if (!d_winrate) return e->fail(LZ_ERR_ARG, "null buffer");
+    int rc = check_positions(e, pos, n);
+    if (rc != LZ_OK) return rc;
+    if (hipSetDevice(e->device) != hipSuccess)
+        return e->fail(LZ_ERR_HIP, "hipSetDevice failed");
+    rc = pattern_setup(e);
+    if (rc != LZ_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: default stream
+    if (e->pat_used && hipStreamWaitEvent(s, e->pat_done, 0) != hipSuccess)
+        return e->fail(LZ_ERR_HIP, "pattern playout wait");
+    lz_encoder::Slot *sl = nullptr;
+    rc = send_records(e, pos, n, s, &sl);
+    if (rc != LZ_OK) return rc;
+    lz::mc::TablesView tv;
+    tv.g = e->pat_g;
+    tv.index[0] = e->pat_index[0];
+    tv.index[1] = e->pat_index[1];
+    tv.feature_weight = e->pat_fw;
+    tv.pattern_weight = e->pat_pw;
+    hipError_t err = lz::launch_pattern_playouts(
+        sl->dev, n, nplay, seed0, tv, e->pat_states, e->pat_nstates,
+        e->half_wins, e->pat_overflow, d_winrate, s);
+    if (err == hipSuccess) err = hipEventRecord(e->pat_done, s);
+    if (err != hipSuccess)
+        return e->fail(LZ_ERR_HIP, "pattern playout launch: %s",
+                       hipGetErrorString(err));
+    e->pat_used = true;
+    return release_slot(e, sl, s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -233,6 +340,21 @@ int lz_playout_winrates_device(lz_encoder *e, const lz_position *const *pos,
     if (!e) return LZ_ERR_ARG;
     std::lock_guard<std::mutex> g(e->mu);
     return playouts(e, pos, n, playouts_per, seed0, d_winrates, stream);
+}
+
+int lz_playout_winrates_device_policy(lz_encoder *e,
+                                      const lz_position *const *pos, int n,
+                                      int playouts_per, uint32_t seed0,
+                                      int policy, float *d_winrates,
+                                      void *stream) {
+    if (!e) return LZ_ERR_ARG;
+    std::lock_guard<std::mutex> g(e->mu);
+    if (policy == LZ_PLAYOUT_LIGHT)
+        return playouts(e, pos, n, playouts_per, seed0, d_winrates, stream);
+    if (policy != LZ_PLAYOUT_PATTERN)
+        return e->fail(LZ_ERR_ARG, "playout policy %d", policy);
+    return pattern_playouts(e, pos, n, playouts_per, seed0, d_winrates,
+                            stream);
 }
 
 int lz_encode_device_both(lz_encoder *e, const lz_position *const *pos, int n,

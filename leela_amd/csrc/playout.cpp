@@ -57,13 +57,7 @@ float playout(GameState &st, Rng &rng) {
 }
 
 uint32_t playout_seed(uint32_t seed, uint32_t j) {
-    uint32_t h = seed ^ (j * 0x9E3779B9u);
-    h ^= h >> 16;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    return h;
+    return lz::playout_seed(seed, j);
 }
 
 float mc_winrate(const GameState &st, int n, uint32_t seed) {

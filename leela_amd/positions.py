@@ -69,6 +69,8 @@ def _lib():
                                             C.c_int, vp, vp, vp]
         L.lz_playout_winrates_device.argtypes = [vp, C.POINTER(vp), C.c_int,
                                                  C.c_int, C.c_uint32, vp, vp]
+        L.lz_playout_winrates_device_policy.argtypes = [
+            vp, C.POINTER(vp), C.c_int, C.c_int, C.c_uint32, C.c_int, vp, vp]
         L._positions_bound = True
     return L
 
@@ -259,29 +261,35 @@ class DeviceEncoder:
                              .decode())
 
     def playouts_into(self, positions, playouts: int, seed0: int,
-                      d_winrates: int, stream: int = 0):
-        """Enqueue the positions' playout win rates (lz_playout_winrate on
-        the GPU, playouts of position i seeded seed0 + i) into device
-        address d_winrates (float per position)."""
+                      d_winrates: int, stream: int = 0,
+                      policy: int = PLAYOUT_LIGHT):
+        """Enqueue the positions' playout win rates (lz_playout_winrate_policy
+        on the GPU, playouts of position i seeded seed0 + i) into device
+        address d_winrates (float per position; NaN where a pattern
+        playout was abandoned)."""
         arr, n = _as_array(positions)
-        rc = self._L.lz_playout_winrates_device(self._h, arr, n, playouts,
-                                                seed0 & 0xFFFFFFFF,
-                                                C.c_void_p(d_winrates),
-                                                C.c_void_p(stream))
+        rc = self._L.lz_playout_winrates_device_policy(
+            self._h, arr, n, playouts, seed0 & 0xFFFFFFFF, policy,
+            C.c_void_p(d_winrates), C.c_void_p(stream))
         if rc != LZ_OK:
             raise LeelaError(rc, self._L.lz_encoder_last_error(self._h)
                              .decode())
 
-    def playout_winrates(self, positions, playouts: int,
-                         seed0: int) -> np.ndarray:
-        """Win rates as a numpy array (synchronous)."""
+    def playout_winrates(self, positions, playouts: int, seed0: int,
+                         policy: int = PLAYOUT_LIGHT) -> np.ndarray:
+        """Win rates as a numpy array (synchronous); raises if a pattern
+        playout outgrew the device's candidate list."""
         import torch
         dev = torch.device("cuda", self.device)
         out = torch.empty(max(len(positions), 1), dtype=torch.float32,
                           device=dev)
         self.playouts_into(positions, playouts, seed0, out.data_ptr(),
-                           torch.cuda.current_stream(dev).cuda_stream)
-        return out[:len(positions)].cpu().numpy()
+                           torch.cuda.current_stream(dev).cuda_stream, policy)
+        w = out[:len(positions)].cpu().numpy()
+        if np.isnan(w).any():
+            raise LeelaError(-5, f"{int(np.isnan(w).sum())} positions with a "
+                             "playout beyond the device candidate list")
+        return w
 
     def encode(self, positions, net: int = NET_POLICY, symmetry: int = 0):
         """Planes as a uint8 CUDA tensor: [n, 32, S*S] for one side, flat
@@ -330,10 +338,10 @@ def value_labels(n: int, seed0: int, playouts: int, playout_seed: int,
 def value_labels_device(n: int, seed0: int, playouts: int, playout_seed: int,
                         side: int = 19, max_moves: int = MAX_MOVES_19,
                         planes: bool = True, device: int = 0,
-                        batch: int = 4096):
+                        batch: int = 4096, policy: int = PLAYOUT_LIGHT):
     """value_labels with the playouts (and value planes) computed on the
-    GPU: lz_playout_winrates_device and lz_encode_device over batches of
-    generator positions. Equal to value_labels bit for bit."""
+    GPU: lz_playout_winrates_device_policy and lz_encode_device over
+    batches of generator positions. Equal to value_labels bit for bit."""
     import torch
     enc = DeviceEncoder(device, max_batch=min(batch, max(n, 1)))
     dev = torch.device("cuda", device)
@@ -347,7 +355,7 @@ def value_labels_device(n: int, seed0: int, playouts: int, playout_seed: int,
             d_w = torch.empty(b - a, dtype=torch.float32, device=dev)
             st = torch.cuda.current_stream(dev).cuda_stream
             enc.playouts_into(pos, playouts, playout_seed + a, d_w.data_ptr(),
-                              st)
+                              st, policy)
             if planes:
                 d_pl = torch.empty((b - a, 32, side * side), dtype=torch.uint8,
                                    device=dev)
@@ -356,6 +364,9 @@ def value_labels_device(n: int, seed0: int, playouts: int, playout_seed: int,
             w[a:b] = d_w.cpu().numpy()
     finally:
         enc.close()
+    if np.isnan(w).any():
+        raise LeelaError(-5, f"{int(np.isnan(w).sum())} positions with a "
+                         "playout beyond the device candidate list")
     return pl, w
 
 
