@@ -20,8 +20,8 @@ surrogate's inference rate at batch `--infer-batch` and the playout rate on
 `--threads` host threads. Prints one JSON line.
 
 `--policy pattern` labels with the reference's pattern playout policy
-(host playouts; every label, training and evaluation, from that teacher);
-the default light policy runs on the GPU.
+(every label, training and evaluation, from that teacher); both policies
+run on the GPU (the same values as on the host).
 """
 from __future__ import annotations
 
@@ -59,7 +59,7 @@ def main(argv=None) -> int:
     ap.add_argument("--policy", choices=("light", "pattern"),
                     default="light",
                     help="playout policy of the labels (pattern: the "
-                         "reference's, on host threads)")
+                         "reference's)")
     ap.add_argument("--host-labels", action="store_true",
                     help="label with host playouts (default: GPU playouts, "
                          "the same values)")
@@ -71,17 +71,17 @@ def main(argv=None) -> int:
 
     policy = (positions.PLAYOUT_PATTERN if args.policy == "pattern"
               else positions.PLAYOUT_LIGHT)
-    host = args.host_labels or policy == positions.PLAYOUT_PATTERN
+    host = args.host_labels
 
     def make_labels(n, seed0, playouts, playout_seed0, planes=True):
-        # GPU playouts unless asked for the host's (equal values); the
-        # pattern policy runs on the host.
+        # GPU playouts unless asked for the host's (equal values).
         if host:
             return positions.value_labels(n, seed0, playouts, playout_seed0,
                                           threads=args.threads, planes=planes,
                                           policy=policy)
         return positions.value_labels_device(n, seed0, playouts,
-                                             playout_seed0, planes=planes)
+                                             playout_seed0, planes=planes,
+                                             policy=policy)
 
     t0 = time.perf_counter()
     if args.labels and os.path.exists(args.labels):

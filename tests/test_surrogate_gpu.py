@@ -307,6 +307,42 @@ def test_training_beats_playouts_at_equal_cost():
     assert t_sur < t_po
 
 
+def test_training_on_the_reference_teacher():
+    """The same gain test with the reference's own playout policy as the
+    teacher (pattern playouts on the GPU, equal to the host's and the
+    reference's): 16,384 positions x 32 pattern playouts for training,
+    1,024 held-out positions judged against a fresh 256-playout mean of the
+    same policy. The surrogate must beat the constant predictor by 20% and
+    a 16-playout estimate of that policy."""
+    import torch
+    from leela_amd.positions import PLAYOUT_PATTERN
+    print("surrogate training: labels 16384 x 32 pattern playouts", flush=True)
+    tr_planes, tr_y = positions.value_labels_device(
+        16384, 30000, 32, 3_000_000, batch=4096, policy=PLAYOUT_PATTERN)
+    ev_planes, ev_y = positions.value_labels_device(
+        1024, 40000, 256, 4_000_000, batch=1024, policy=PLAYOUT_PATTERN)
+    _, est16 = positions.value_labels_device(
+        1024, 40000, 16, 5_000_000, batch=1024, planes=False,
+        policy=PLAYOUT_PATTERN)
+    s = Surrogate(0, max_batch=1024, seed=7)
+    try:
+        print("surrogate training: 600 steps, batch 256", flush=True)
+        losses = train(s, tr_planes, tr_y, steps=600, batch=256, lr=1e-3,
+                       seed=1, log_every=100)
+        assert losses[-1][1] < 0.8 * losses[0][1]
+        pred = s.predict(ev_planes)
+        torch.cuda.synchronize()
+    finally:
+        s.close()
+    mae_sur = float(np.mean(np.abs(pred - ev_y)))
+    mae_const = float(np.mean(np.abs(tr_y.mean() - ev_y)))
+    mae_16 = float(np.mean(np.abs(est16 - ev_y)))
+    print(f"pattern teacher: MAE surrogate {mae_sur:.4f} constant "
+          f"{mae_const:.4f} 16-playout {mae_16:.4f}", flush=True)
+    assert mae_sur < 0.8 * mae_const
+    assert mae_sur < mae_16
+
+
 def test_config4_at_full_scale(capsys):
     """The surrogate workload at its stated scale, through the training CLI:
     200,000 generator positions (seeds 10000.., move counts 0..250) labelled
