@@ -97,6 +97,16 @@ int lz_push_batchnorm(lz_ctx *ctx, int net, int channels, const float *means,
  * packing for MFMA needs the whole table, hence the extra step. */
 int lz_finalize(lz_ctx *ctx);
 
+/* Batches of at most max_boards 19x19 positions (0..64; default 64) run
+ * the 16-bit forms layered: one launch per conv, each conv of a board split
+ * over several workgroups, instead of one workgroup per board running the
+ * whole chain. Same outputs bit for bit; lower latency for small batches
+ * (the leaf queue's smallest bucket, synchronous single-position calls).
+ * 0 keeps every batch on the persistent kernel. Launch-shape setting with
+ * no reference counterpart; applies to forwards and queue graphs captured
+ * after the call. */
+int lz_set_layered_max(lz_ctx *ctx, int max_boards);
+
 /* Softmax temperature of the policy head (cfg_softmax_temp, GTP.cpp:89-93;
  * default 0.75). */
 int lz_set_softmax_temp(lz_ctx *ctx, float temperature);

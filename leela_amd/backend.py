@@ -74,6 +74,7 @@ def load_library():
                                     C.c_float]
     L.lz_finalize.argtypes = [vp]
     L.lz_set_softmax_temp.argtypes = [vp, C.c_float]
+    L.lz_set_layered_max.argtypes = [vp, C.c_int]
     L.lz_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int,
                              C.POINTER(C.c_uint8), f32p, f32p]
     L.lz_forward_device.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp,
@@ -198,6 +199,12 @@ class Backend:
 
     def finalize(self):
         self._check(self._L.lz_finalize(self._ctx), "lz_finalize")
+
+    def set_layered_max(self, max_boards: int):
+        """lz_set_layered_max: batches of at most max_boards 19x19
+        positions run the layered 16-bit form (0: never)."""
+        self._check(self._L.lz_set_layered_max(self._ctx, max_boards),
+                    "lz_set_layered_max")
 
     def set_temperature(self, t: float):
         self._check(self._L.lz_set_softmax_temp(self._ctx, t),

@@ -126,14 +126,15 @@ __device__ __forceinline__ float elu_operand(float v) {
 }
 
 // One same-padded conv of the tile `act` (row pitch rp elements, pixel
-// stride PS, CIN input channels) into acc[MTW][kNT]: m-tiles mt0..mt0+MTW-1 (of MT in the
+// stride PS, CIN input channels) into acc[MTW][NT]: m-tiles mt0..mt0+MTW-1 (of MT in the
 // packed weights) for the wave's column tiles; pb[t] is the lane's window
 // origin in column tile t.
-template <typename T, int F, int CIN, int MT, int MTW, int PS>
+template <typename T, int F, int CIN, int MT, int MTW, int PS,
+          int NT = kNT>
 __device__ __forceinline__ void conv_tile(const T *act,
                                           const typename Op<T>::v8 *__restrict__ wpk,
-                                          f32x4 (&acc)[MTW][kNT],
-                                          const int (&pb)[kNT], int rp,
+                                          f32x4 (&acc)[MTW][NT],
+                                          const int (&pb)[NT], int rp,
                                           int mt0, int lane,
                                           const float *__restrict__ bias =
                                               nullptr) {
@@ -152,7 +153,7 @@ __device__ __forceinline__ void conv_tile(const T *act,
             for (int r = 0; r < 4; r++) init[r] = bias[o0 + r];
         }
 #pragma unroll
-        for (int t = 0; t < kNT; t++) acc[m][t] = init;
+        for (int t = 0; t < NT; t++) acc[m][t] = init;
     }
 
     // (The empty asm keeps the compiler from hoisting the per-step
@@ -176,40 +177,40 @@ __device__ __forceinline__ void conv_tile(const T *act,
     // B fragments (LDS) are loaded one k-step ahead into the other of two
     // register sets; the step loop is unrolled by two so that the sets
     // alternate without copies.
-    v8 a[MTW], b0[kNT], b1[kNT];
+    v8 a[MTW], b0[NT], b1[NT];
 #pragma unroll
     for (int m = 0; m < MTW; m++) a[m] = a_at(0, m);
 #pragma unroll
-    for (int t = 0; t < kNT; t++)
+    for (int t = 0; t < NT; t++)
         b0[t] = *reinterpret_cast<const v8 *>(act + pb[t] + g8);
 
     // Step s: acc += a * b while the B fragments of step s + 1 load into
     // bn. The MFMAs run m-tile by m-tile at raised wave priority; each A
     // fragment reloads with step s + 1's right after its last MFMA (a full
     // step of MFMAs ahead of its next use: the loads come from L2), and
-    // the kNT LDS reads are spread over the first half of the MFMAs.
-    auto step = [&](const v8(&b)[kNT], v8(&bn)[kNT], int s) {
+    // the NT LDS reads are spread over the first half of the MFMAs.
+    auto step = [&](const v8(&b)[NT], v8(&bn)[NT], int s) {
         const int off = b_off(s + 1);
 #pragma unroll
-        for (int t = 0; t < kNT; t++)
+        for (int t = 0; t < NT; t++)
             bn[t] = *reinterpret_cast<const v8 *>(act + pb[t] + off);
         __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int m = 0; m < MTW; m++) {
 #pragma unroll
-            for (int t = 0; t < kNT; t++)
+            for (int t = 0; t < NT; t++)
                 acc[m][t] = Op<T>::mfma(a[m], b[t], acc[m][t]);
             a[m] = a_at(s + 1, m);
         }
-        constexpr int kMfma = MTW * kNT;
-        constexpr int kDsEvery = (kMfma / 2) / kNT > 0 ? (kMfma / 2) / kNT : 1;
+        constexpr int kMfma = MTW * NT;
+        constexpr int kDsEvery = (kMfma / 2) / NT > 0 ? (kMfma / 2) / NT : 1;
         int ds = 0;
 #pragma unroll
         for (int m = 0; m < MTW; m++) {
 #pragma unroll
-            for (int t = 0; t < kNT; t++) {
+            for (int t = 0; t < NT; t++) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-                if ((m * kNT + t) % kDsEvery == kDsEvery - 1 && ds < kNT) {
+                if ((m * NT + t) % kDsEvery == kDsEvery - 1 && ds < NT) {
                     __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
                     ds++;
                 }
@@ -228,10 +229,10 @@ __device__ __forceinline__ void conv_tile(const T *act,
 // written into the trunk tile at the lane's pixel offsets po[t] (-1:
 // padding column). D layout of 16x16 MFMA: lane holds rows 4*(lane/16)+r
 // of column lane%16, i.e. four consecutive output channels of one pixel.
-template <typename T, int MTW>
+template <typename T, int MTW, int NT = kNT>
 __device__ __forceinline__ void store_tile(T *act,
-                                           const f32x4 (&acc)[MTW][kNT],
-                                           int mt0, const int (&po)[kNT],
+                                           const f32x4 (&acc)[MTW][NT],
+                                           int mt0, const int (&po)[NT],
                                            int lane) {
     typedef typename Op<T>::v4 v4;
     const int g4 = (lane >> 4) * 4;
@@ -239,7 +240,7 @@ __device__ __forceinline__ void store_tile(T *act,
     for (int m = 0; m < MTW; m++) {
         const int o0 = (mt0 + m) * 16 + g4;
 #pragma unroll
-        for (int t = 0; t < kNT; t++) {
+        for (int t = 0; t < NT; t++) {
             if (po[t] < 0) continue;
             v4 v;
 #pragma unroll
@@ -761,6 +762,302 @@ hipError_t launch(int which, int wm, const uint8_t *planes,
                                              grid, stream);
 }
 
+// ---- Layered path: small batches of 19x19 boards ---------------------------
+// The persistent kernel runs a board's whole layer chain on one CU (~0.2 ms
+// for the policy net), which is the latency of a small leaf batch whatever
+// its size. The layered path spreads each conv of a board over several
+// workgroups (groups of kLMT output m-tiles), one launch per conv, with the
+// activations between convs in global memory in the LDS tile layout
+// (zero halos written once, never overwritten). Every output is formed by
+// the same MFMA sequence from the same operands, the same bias start,
+// activation and rounding as in fused_net, so the two paths agree bit for
+// bit.
+constexpr int kLS = 19;
+constexpr int kLMT = 2;  // output m-tiles per workgroup
+template <int PS>
+constexpr int lrp() { return (kLS + 2) * PS + row_pad(PS, 2); }
+constexpr int lrp_in() { return (kLS + 4) * kInPS + row_pad(kInPS, 4); }
+// A board's trunk tile, rounded up to whole KiB (512 16-bit elements) so
+// that it moves as KiB chunks.
+template <int PS>
+constexpr int ltile() {
+    return ((1 + (kLS + 1)) * lrp<PS>() + 511) / 512 * 512;
+}
+constexpr int ltile_in() { return (2 + (kLS + 2)) * lrp_in(); }
+constexpr int kLPlaneBytes = 32 * kP;  // a board's input planes
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef const __attribute__((address_space(1))) void gbl_void_t;
+
+// Copies `chunks` KiB from global memory to LDS (both 16-byte aligned) with
+// direct global-to-LDS loads, 16 bytes per lane (no registers, all in
+// flight at once); the waves take chunks round robin. The caller waits
+// (vmcnt(0)) and synchronises the workgroup.
+__device__ __forceinline__ void copy_to_lds(const void *src, void *dst,
+                                            int chunks, int wave, int lane) {
+    const char *g = static_cast<const char *>(src) + lane * 16;
+    char *l = static_cast<char *>(dst);
+    for (int c = wave; c < chunks; c += kWN)
+        __builtin_amdgcn_global_load_lds((gbl_void_t *)(g + c * 1024),
+                                         (lds_void_t *)(l + c * 1024), 16, 0,
+                                         0);
+}
+
+__device__ __forceinline__ void wait_lds_copies() {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// fused_net's per-lane pixel offsets for one 19x19 board (k = 1).
+template <int PS>
+__device__ __forceinline__ void lgeom(int nt0, int lane, int (&pb_in)[kNT],
+                                      int (&pb)[kNT], int (&po)[kNT],
+                                      int (&pix)[kNT]) {
+    constexpr int S = kLS, SS = S * S, npix = SS;
+    constexpr int rp_in = lrp_in(), rp = lrp<PS>();
+#pragma unroll
+    for (int t = 0; t < kNT; t++) {
+        const int j = (nt0 + t) * 16 + (lane & 15);
+        pix[t] = j < npix ? j : -1;
+        const int jj = j < npix ? j : j - ((j - npix) / 16 + 1) * 16;
+        const int y = jj / S, x = jj % S;
+        pb_in[t] = y * rp_in + x * kInPS;
+        pb[t] = y * rp + x * PS;
+        po[t] = j < npix ? pb[t] + rp + PS : -1;
+    }
+}
+
+__device__ __forceinline__ int layered_boards(int n, const int *counts) {
+    return counts ? min(n, counts[1]) : n;
+}
+
+// Stem: 5x5, 32 planes -> M1, written into the trunk tile of the board.
+template <typename T, int PS, int M1>
+__global__ __launch_bounds__(kWN * 64) void layered_stem(
+    const uint8_t *__restrict__ planes, const void *w,
+    const float *__restrict__ bias, T *__restrict__ out_tiles, int n,
+    const int *counts) {
+    typedef typename Op<T>::v8 v8;
+    constexpr int MT1 = M1 / 16, G = MT1 / kLMT;
+    constexpr int S = kLS, pin = S + 4, rp_in = lrp_in();
+    static_assert(MT1 % kLMT == 0, "stem channel groups");
+    const int board = blockIdx.x / G, cg = blockIdx.x % G;
+    if (board >= layered_boards(n, counts)) return;
+    __shared__ __attribute__((aligned(16))) T s_act[ltile_in()];
+    __shared__ __attribute__((aligned(16))) uint8_t s_pl[kLPlaneBytes];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // The board's planes into LDS: whole KiB chunks directly, the 288-byte
+    // tail through registers (no read past the board).
+    const uint8_t *src = planes + (size_t)board * kLPlaneBytes;
+    constexpr int kFull = kLPlaneBytes / 1024, kTail = kLPlaneBytes % 1024;
+    static_assert(kTail % 16 == 0 && kLPlaneBytes % 16 == 0, "plane copy");
+    copy_to_lds(src, s_pl, kFull, wave, lane);
+    if (tid < kTail / 16)
+        reinterpret_cast<uint4 *>(s_pl + kFull * 1024)[tid] =
+            reinterpret_cast<const uint4 *>(src + kFull * 1024)[tid];
+    int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
+    lgeom<PS>(wave * kNT, lane, pb_in, pb, po, pix);
+    wait_lds_copies();
+    __syncthreads();
+    // The 32 binary planes with a 2-wide zero halo (as fused_net stages
+    // them).
+    constexpr int rows_in = 2 + (S + 2);
+    for (int i = tid; i < rows_in * pin * 4; i += kWN * 64) {
+        const int pp = i >> 2, cg8 = (i & 3) * 8;
+        const int row = pp / pin - 2, xx = pp % pin - 2;
+        const bool in = row >= 0 && row < S && xx >= 0 && xx < S;
+        v8 v;
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+            v[e] = (T)(in ? (float)s_pl[(cg8 + e) * kP + row * S + xx] : 0.0f);
+        *reinterpret_cast<v8 *>(s_act + (row + 2) * rp_in + (xx + 2) * kInPS +
+                                cg8) = v;
+    }
+    __syncthreads();
+    f32x4 acc[kLMT][kNT];
+    conv_tile<T, 5, 32, MT1, kLMT, kInPS>(s_act, static_cast<const v8 *>(w),
+                                          acc, pb_in, rp_in, cg * kLMT, lane,
+                                          bias);
+    store_tile<T, kLMT>(out_tiles + (size_t)board * ltile<PS>(), acc,
+                        cg * kLMT, po, lane);
+}
+
+// Row bands: a 3x3 conv of a board is split into kBands workgroups of
+// kBandRows image rows (the last band the rest), each staging only its rows
+// and their two halo rows of the input tile and computing every output
+// channel of its pixels: four waves over the m-tiles, kBandNT column tiles
+// of 16 of the band's pixels each.
+constexpr int kBandRows = 4;
+constexpr int kBands = (kLS + kBandRows - 1) / kBandRows;
+constexpr int kBandNT = (kBandRows * kLS + 15) / 16;
+
+// Band b's pixel offsets: window origins in the staged rows (pb), output
+// offsets in the board's global tile (po, -1 past the band), pixel index
+// on the board (pix, -1 past the band).
+template <int PS>
+__device__ __forceinline__ void band_geom(int b, int lane,
+                                          int (&pb)[kBandNT],
+                                          int (&po)[kBandNT],
+                                          int (&pix)[kBandNT]) {
+    constexpr int S = kLS, rp = lrp<PS>();
+    const int y0 = b * kBandRows;
+    const int rows = min(kBandRows, S - y0), npix = rows * S;
+#pragma unroll
+    for (int t = 0; t < kBandNT; t++) {
+        const int j = t * 16 + (lane & 15);
+        // A lane past the band reads a pixel a multiple of 16 before it.
+        const int jj = j < npix ? j : j - ((j - npix) / 16 + 1) * 16;
+        const int yl = jj / S, x = jj % S;
+        pb[t] = yl * rp + x * PS;
+        po[t] = j < npix ? (y0 + yl + 1) * rp + (x + 1) * PS : -1;
+        pix[t] = j < npix ? (y0 + yl) * S + x : -1;
+    }
+}
+
+// Stages band b's rows y0 - 1 .. y0 + rows of the board's tile (global
+// rows y0 .. y0 + rows + 1) into LDS: whole KiB chunks directly, the tail
+// through registers.
+template <typename T, int PS>
+__device__ __forceinline__ void band_stage(const T *tile, int b, T *s_act,
+                                           int tid, int wave, int lane) {
+    constexpr int rp = lrp<PS>();
+    const int y0 = b * kBandRows;
+    const int rows = min(kBandRows, kLS - y0) + 2;
+    const char *src = reinterpret_cast<const char *>(tile + (size_t)y0 * rp);
+    const int bytes = rows * rp * (int)sizeof(T);
+    const int full = bytes / 1024, tail = bytes % 1024;
+    copy_to_lds(src, s_act, full, wave, lane);
+    if (tid * 16 < tail)
+        reinterpret_cast<uint4 *>(reinterpret_cast<char *>(s_act) +
+                                  full * 1024)[tid] =
+            reinterpret_cast<const uint4 *>(src + full * 1024)[tid];
+}
+
+// One 3x3 conv CIN -> C of the trunk for one band of one board.
+template <typename T, int PS, int CIN, int C>
+__global__ __launch_bounds__(kWN * 64) void layered_conv(
+    const T *__restrict__ in_tiles, const void *w,
+    const float *__restrict__ bias, T *__restrict__ out_tiles, int n,
+    const int *counts) {
+    typedef typename Op<T>::v8 v8;
+    constexpr int MT = C / 16, MTW = MT / kWN, TILE = ltile<PS>();
+    static_assert(MT % kWN == 0 && TILE % 512 == 0, "trunk tiling");
+    static_assert((lrp<PS>() * (int)sizeof(T)) % 16 == 0 &&
+                      (kBandRows + 2) * lrp<PS>() * (int)sizeof(T) % 1024 <=
+                          kWN * 64 * 16,
+                  "band staging");
+    const int board = blockIdx.x / kBands, b = blockIdx.x % kBands;
+    if (board >= layered_boards(n, counts)) return;
+    __shared__ __attribute__((aligned(16))) T s_act[(kBandRows + 2) * lrp<PS>()];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    band_stage<T, PS>(in_tiles + (size_t)board * TILE, b, s_act, tid, wave,
+                      lane);
+    int pb[kBandNT], po[kBandNT], pix[kBandNT];
+    band_geom<PS>(b, lane, pb, po, pix);
+    wait_lds_copies();
+    __syncthreads();
+    f32x4 acc[MTW][kBandNT];
+    conv_tile<T, 3, CIN, MT, MTW, PS, kBandNT>(
+        s_act, static_cast<const v8 *>(w), acc, pb, lrp<PS>(), wave * MTW,
+        lane, bias);
+    store_tile<T, MTW, kBandNT>(out_tiles + (size_t)board * TILE, acc,
+                                wave * MTW, po, lane);
+}
+
+// The 1-channel head conv (FP32 accumulation, ELU) of one band of one
+// board, by wave 0, into plane [n][361].
+template <typename T, int PS, int C>
+__global__ __launch_bounds__(kWN * 64) void layered_head(
+    const T *__restrict__ in_tiles, const void *w,
+    const float *__restrict__ hbias, float *__restrict__ plane, int n,
+    const int *counts) {
+    typedef typename Op<T>::v8 v8;
+    constexpr int TILE = ltile<PS>();
+    const int board = blockIdx.x / kBands, b = blockIdx.x % kBands;
+    if (board >= layered_boards(n, counts)) return;
+    __shared__ __attribute__((aligned(16))) T s_act[(kBandRows + 2) * lrp<PS>()];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    band_stage<T, PS>(in_tiles + (size_t)board * TILE, b, s_act, tid, wave,
+                      lane);
+    int pb[kBandNT], po[kBandNT], pix[kBandNT];
+    band_geom<PS>(b, lane, pb, po, pix);
+    wait_lds_copies();
+    __syncthreads();
+    if (wave != 0) return;
+    f32x4 acc[1][kBandNT];
+    conv_tile<T, 3, C, 1, 1, PS, kBandNT>(s_act, static_cast<const v8 *>(w),
+                                          acc, pb, lrp<PS>(), 0, lane);
+    const float hb = hbias[0];
+    if (lane < 16) {
+#pragma unroll
+        for (int t = 0; t < kBandNT; t++)
+            if (pix[t] >= 0)
+                plane[(size_t)board * kP + pix[t]] = elu(acc[0][t][0] + hb);
+    }
+}
+
+// softmax(T) of each board's head plane (Network.cpp:450-469), one wave
+// per board.
+__global__ __launch_bounds__(64) void layered_softmax(
+    const float *__restrict__ plane, float *__restrict__ raw,
+    float *__restrict__ out, int n, const int *counts, float temperature) {
+    const int board = blockIdx.x;
+    if (board >= layered_boards(n, counts)) return;
+    const size_t o = (size_t)board * kP;
+    wave_softmax(plane + o, kP, temperature, raw ? raw + o : nullptr,
+                 out ? out + o : nullptr, threadIdx.x);
+}
+
+template <typename T>
+hipError_t layered_run(int which, const uint8_t *planes, const FusedArgs &a,
+                       float *raw, float *out, int n, float temperature,
+                       void *tiles, hipStream_t s) {
+    const dim3 blk(kWN * 64);
+    // Two policy tiles per board, then two value tiles: each net keeps its
+    // own layout's zero halos.
+    T *pol = static_cast<T *>(tiles);
+    T *val = pol + 2 * (size_t)kLayeredMaxBoards * ltile<144>();
+    T *buf[2];
+    if (which == 0) {
+        buf[0] = pol;
+        buf[1] = pol + (size_t)kLayeredMaxBoards * ltile<144>();
+    } else {
+        buf[0] = val;
+        buf[1] = val + (size_t)kLayeredMaxBoards * ltile<80>();
+    }
+    const int *cnt = a.counts;
+    if (!a.h0) return hipErrorInvalidValue;  // the head planes
+    if (which == 0) {
+        if (a.nconv != 13 || a.c != 128 || a.m1 != 96)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((layered_stem<T, 144, 96>), dim3(n * 3), blk, 0, s,
+                           planes, a.w[0], a.b[0], buf[0], n, cnt);
+        hipLaunchKernelGGL((layered_conv<T, 144, 96, 128>), dim3(n * kBands),
+                           blk, 0, s, buf[0], a.w[1], a.b[1], buf[1], n, cnt);
+        for (int l = 2; l <= 11; l++)
+            hipLaunchKernelGGL((layered_conv<T, 144, 128, 128>),
+                               dim3(n * kBands), blk, 0, s, buf[(l + 1) & 1],
+                               a.w[l], a.b[l], buf[l & 1], n, cnt);
+        hipLaunchKernelGGL((layered_head<T, 144, 128>), dim3(n * kBands), blk,
+                           0, s, buf[1], a.w[12], a.b[12], a.h0, n, cnt);
+        hipLaunchKernelGGL(layered_softmax, dim3(n), dim3(64), 0, s, a.h0,
+                           raw, out, n, cnt, temperature);
+        return hipGetLastError();
+    }
+    if (a.nconv != 12 || a.c != 64 || a.m1 != 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((layered_stem<T, 80, 64>), dim3(n * 2), blk, 0, s,
+                       planes, a.w[0], a.b[0], buf[0], n, cnt);
+    for (int l = 1; l <= 10; l++)
+        hipLaunchKernelGGL((layered_conv<T, 80, 64, 64>), dim3(n * kBands),
+                           blk, 0, s, buf[(l + 1) & 1], a.w[l], a.b[l],
+                           buf[l & 1], n, cnt);
+    hipLaunchKernelGGL((layered_head<T, 80, 64>), dim3(n * kBands), blk, 0, s,
+                       buf[0], a.w[11], a.b[11], a.h0, n, cnt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return value_fc_head(a.h0, a.fc1_wt, a.fc1_b, a.fc2_w, a.fc2_b, raw, out,
+                         n, kP, a.hidden, s, cnt ? cnt + 1 : nullptr);
+}
+
 }  // namespace
 
 int fused_default_wm(int which) { return which == 0 ? 2 : 1; }
@@ -784,6 +1081,27 @@ int fused_blocks_per_cu(int which, int wm) {
 }
 
 int ragged_slot_capacity(int side) { return slot_capacity(side); }
+
+size_t layered_tile_bytes() {
+    return 2 * (size_t)kLayeredMaxBoards * (ltile<144>() + ltile<80>()) *
+           sizeof(uint16_t);
+}
+
+hipError_t fused_forward_layered(int which, FusedType type,
+                                 const uint8_t *planes, const FusedArgs &args,
+                                 float *raw, float *out, int n,
+                                 float temperature, void *tiles,
+                                 hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    if (n > kLayeredMaxBoards || !tiles || args.slots ||
+        type == FusedType::kFP8)
+        return hipErrorInvalidValue;
+    return type == FusedType::kBF16
+               ? layered_run<__bf16>(which, planes, args, raw, out, n,
+                                     temperature, tiles, stream)
+               : layered_run<_Float16>(which, planes, args, raw, out, n,
+                                       temperature, tiles, stream);
+}
 
 hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
                          const FusedArgs &args, float *raw, float *out, int n,

@@ -236,6 +236,16 @@ int lz_finalize(lz_ctx *ctx) {
     return LZ_OK;
 }
 
+int lz_set_layered_max(lz_ctx *ctx, int max_boards) {
+    if (!ctx) return LZ_ERR_ARG;
+    if (max_boards < 0 || max_boards > lz::kLayeredMaxBoards)
+        return ctx->fail(LZ_ERR_ARG, "layered batch limit %d outside 0..%d",
+                         max_boards, lz::kLayeredMaxBoards);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    for (auto &n : ctx->net) n.layered_max = max_boards;
+    return LZ_OK;
+}
+
 int lz_set_softmax_temp(lz_ctx *ctx, float temperature) {
     if (!ctx) return LZ_ERR_ARG;
     if (!(temperature > 0.0f) || !std::isfinite(temperature))

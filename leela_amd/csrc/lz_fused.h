@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace lz {
@@ -76,5 +77,19 @@ hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
                          const FusedArgs &args, float *raw, float *out, int n,
                          float temperature, const FusedLaunch &cfg,
                          hipStream_t stream);
+
+// Layered form of fused_forward for at most kLayeredMaxBoards 19x19 boards
+// (BF16 / FP16, not ragged): one launch per conv, each conv of a board
+// split over several workgroups, activations between convs in `tiles`
+// (layered_tile_bytes(), zeroed once before first use: the tiles' halos
+// must stay zero). Bitwise the same outputs as fused_forward, at a lower
+// latency for small batches.
+constexpr int kLayeredMaxBoards = 64;
+size_t layered_tile_bytes();
+hipError_t fused_forward_layered(int which, FusedType type,
+                                 const uint8_t *planes, const FusedArgs &args,
+                                 float *raw, float *out, int n,
+                                 float temperature, void *tiles,
+                                 hipStream_t stream);
 
 }  // namespace lz

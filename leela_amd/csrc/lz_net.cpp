@@ -348,6 +348,20 @@ int ensure_workspace(Workspace &ws, const Net nets[2], int n,
         return LZ_ERR_NOMEM;
     }
     ws.cap = n;
+    return ensure_tiles(ws, msg);
+}
+
+int ensure_tiles(Workspace &ws, std::string *msg) {
+    if (ws.tiles) return LZ_OK;
+    const size_t bytes = layered_tile_bytes();
+    hipError_t e = hipMalloc(&ws.tiles, bytes);
+    if (e == hipSuccess) e = hipMemset(ws.tiles, 0, bytes);
+    if (e != hipSuccess) {
+        if (ws.tiles) (void)hipFree(ws.tiles);
+        ws.tiles = nullptr;
+        *msg = std::string("layered tiles: ") + hipGetErrorString(e);
+        return LZ_ERR_NOMEM;
+    }
     return LZ_OK;
 }
 
@@ -358,6 +372,7 @@ void free_workspace(Workspace &ws) {
     dfree(ws.raw);
     dfree(ws.out);
     dfree(ws.h0);
+    if (ws.tiles) (void)hipFree(ws.tiles);
     ws = Workspace();
 }
 
@@ -469,10 +484,18 @@ int run_net(const Net &net, int which, int precision, int n,
         }
         const FusedLaunch cfg =
             fused_launch_for(net, which, ragged ? ragged->nslots : n);
-        if (e == hipSuccess)
-            e = fused_forward(which, bf ? FusedType::kBF16 : FusedType::kFP16,
-                              d_planes, args, d_raw, d_out, n, temperature,
-                              cfg, stream);
+        const FusedType ft = bf ? FusedType::kBF16 : FusedType::kFP16;
+        if (e == hipSuccess) {
+            // (The layered stem copies planes 16 bytes at a time.)
+            if (!ragged && ws.tiles && n <= net.layered_max &&
+                reinterpret_cast<uintptr_t>(d_planes) % 16 == 0)
+                e = fused_forward_layered(which, ft, d_planes, args, d_raw,
+                                          d_out, n, temperature, ws.tiles,
+                                          stream);
+            else
+                e = fused_forward(which, ft, d_planes, args, d_raw, d_out, n,
+                                  temperature, cfg, stream);
+        }
         if (e == hipSuccess && timer) e = timer->mark(stream);
         if (e != hipSuccess) {
             *msg = std::string("16-bit forward launch: ") +

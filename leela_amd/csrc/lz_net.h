@@ -54,6 +54,9 @@ struct Net {
     FusedArgs fused;        // device pointers for the BF16 fused kernel
     FusedArgs fused16;      // device pointers for the FP16 fused kernel
     FusedArgs fused8;       // FP8 form: FP16 stem, e4m3 later convs
+    // Batches of at most this many 19x19 boards run the layered form of
+    // the 16-bit kernels (fused_forward_layered); 0 disables it.
+    int layered_max = kLayeredMaxBoards;
     int cus = 0;            // compute units of the device
     int occ[3] = {0, 0, 0}; // resident workgroups per CU for wm = 1, 2
     // Launch shape per batch bucket (<=64, <=256, <=1024, more), set by
@@ -76,7 +79,11 @@ struct Workspace {
     float *raw = nullptr;
     float *out = nullptr;
     float *h0 = nullptr;     // value head plane [cap][361]
+    void *tiles = nullptr;   // layered form's activation tiles (zeroed)
 };
+
+// Allocates and zeroes ws.tiles if absent (layered_tile_bytes()).
+int ensure_tiles(Workspace &ws, std::string *msg);
 
 // Device copy of a ragged batch's layout: slots and per-board offsets.
 struct RaggedLayout {

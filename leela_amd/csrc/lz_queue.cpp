@@ -297,9 +297,12 @@ private:
             e = hipMalloc(reinterpret_cast<void **>(&b.ws.h0),
                           sizeof(float) * cap_ * kOut);
             b.ws.cap = cap_;
-            rc = e == hipSuccess ? LZ_OK : LZ_ERR_NOMEM;
-            if (rc != LZ_OK)
+            if (e != hipSuccess) {
+                rc = LZ_ERR_NOMEM;
                 *msg = std::string("queue workspace: ") + hipGetErrorString(e);
+            } else {
+                rc = ensure_tiles(b.ws, msg);
+            }
         }
         if (rc != LZ_OK) return rc;
         // One graph per net, bucket and layout (dense 19x19 / ragged), on
