@@ -99,6 +99,10 @@ struct Batch {
 };
 }  // namespace
 
+// How long the dispatcher polls for a batch that is filling before it
+// sleeps on its condition variable.
+constexpr auto kDispatchPoll = std::chrono::microseconds(500);
+
 class LeafQueue {
 public:
     LeafQueue(lz_ctx *ctx, int precision, int max_batch, int max_wait_us)
@@ -404,10 +408,15 @@ private:
         hipSetDevice(ctx_->device);
         for (;;) {
             Batch *b = nullptr;
+            // While rows are coming in, the dispatcher polls (yielding its
+            // core) rather than sleeping: a wake-up costs tens of
+            // microseconds, as much as a small batch's launch.
+            const auto poll_until = Clock::now() + kDispatchPoll;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 for (;;) {
-                    const auto next = close_due(Clock::now());
+                    const auto now = Clock::now();
+                    const auto next = close_due(now);
                     bool pending = false;
                     for (auto &c : batches_) {
                         if (c.state != Batch::kClosed) continue;
@@ -421,6 +430,11 @@ private:
                     if (!pending && !open_rows) {
                         flush_ = false;
                         if (stopping_) return;
+                    } else if (now < poll_until && !stopping_) {
+                        lk.unlock();
+                        std::this_thread::yield();
+                        lk.lock();
+                        continue;
                     }
                     if (next != Clock::time_point::max())
                         cv_.wait_until(lk, next);
@@ -481,8 +495,21 @@ private:
         if (e == hipSuccess)
             e = hipMemcpyAsync(b.d_planes, b.h_planes, (size_t)n * kPos,
                                hipMemcpyHostToDevice, b.stream);
-        if (e == hipSuccess)
+        // Small dense batches run the layered kernels, launched directly:
+        // a graph of their ~14 nodes costs more host time to launch
+        // (~7 us a node) than the launches themselves, which overlap the
+        // kernels already running.
+        const Net &net = ctx_->net[b.net];
+        if (e == hipSuccess && !ragged && n <= net.layered_max &&
+            b.ws.tiles) {
+            std::string msg;
+            if (run_net(net, b.net, precision_, n, b.d_planes, b.d_raw,
+                        b.d_out, ctx_->temperature, b.ws, b.stream,
+                        &msg) != LZ_OK)
+                e = hipErrorLaunchFailure;
+        } else if (e == hipSuccess) {
             e = hipGraphLaunch(b.graph[b.net][k][ragged ? 1 : 0], b.stream);
+        }
         if (e == hipSuccess)
             e = hipMemcpyAsync(b.h_raw, b.d_raw, sizeof(float) * per * n,
                                hipMemcpyDeviceToHost, b.stream);

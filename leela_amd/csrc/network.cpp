@@ -3,6 +3,7 @@
 #include "network.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -84,12 +85,15 @@ void prune_ladders(const go::GameState &state, const uint8_t *ladder,
 
 namespace {
 
-// One synchronous call's rows in flight through the leaf queue.
+// One synchronous call's rows in flight through the leaf queue. The
+// caller first polls for a while (yielding its core), then sleeps: a leaf
+// batch comes back within a few hundred microseconds, and a sleeping
+// thread's wake-up alone costs tens of them.
 struct SyncCall {
     std::mutex m;
     std::condition_variable cv;
-    int left = 0;
-    int status = LZ_OK;
+    std::atomic<int> left{0};
+    std::atomic<int> status{LZ_OK};
     float *out = nullptr;
     int width = 0;
 };
@@ -99,15 +103,27 @@ struct SyncRow {
     int row;
 };
 
+constexpr auto kSyncPoll = std::chrono::milliseconds(2);
+
 void sync_done(void *user, int status, const float *raw, const float *out) {
     (void)raw;
     SyncRow *r = static_cast<SyncRow *>(user);
     SyncCall *c = r->call;
     if (status == LZ_OK)
         std::copy(out, out + c->width, c->out + (size_t)r->row * c->width);
+    // The count drops under the mutex, and the caller takes the mutex
+    // before it returns: c is not touched once the caller may have left.
     std::lock_guard<std::mutex> g(c->m);
-    if (status != LZ_OK) c->status = status;
-    if (--c->left == 0) c->cv.notify_all();
+    if (status != LZ_OK) c->status.store(status);
+    if (c->left.fetch_sub(1) == 1) c->cv.notify_all();
+}
+
+void wait_rows(SyncCall &c) {
+    const auto until = std::chrono::steady_clock::now() + kSyncPoll;
+    while (c.left.load() != 0 && std::chrono::steady_clock::now() < until)
+        std::this_thread::yield();
+    std::unique_lock<std::mutex> lk(c.m);
+    c.cv.wait(lk, [&] { return c.left.load() == 0; });
 }
 
 }  // namespace
@@ -124,19 +140,18 @@ int Network::forward_rows(int net, int rows, const uint8_t *in, float *out) {
     call.width = net == LZ_NET_POLICY ? kP : 1;
     std::vector<SyncRow> row(rows);
     int submitted = 0, rc = LZ_OK;
-    {
-        std::lock_guard<std::mutex> g(call.m);
-        call.left = rows;
-    }
+    call.left.store(rows);
     for (int r = 0; r < rows && rc == LZ_OK; r++) {
         row[r] = {&call, r};
         rc = lz_submit(ctx_, net, in + (size_t)r * kPos, sync_done, &row[r]);
         if (rc == LZ_OK) submitted++;
     }
-    std::unique_lock<std::mutex> lk(call.m);
-    call.left -= rows - submitted;  // rows never queued
-    call.cv.wait(lk, [&] { return call.left == 0; });
-    return rc != LZ_OK ? rc : call.status;
+    {
+        std::lock_guard<std::mutex> g(call.m);
+        call.left.fetch_sub(rows - submitted);  // rows never queued
+    }
+    wait_rows(call);
+    return rc != LZ_OK ? rc : call.status.load();
 }
 
 void Network::benchmark(const go::GameState &state, int threads,
