@@ -549,9 +549,15 @@ int lz_make_value_labels_policy(int policy, int n, int side, uint32_t seed0,
                                 uint32_t playout_seed, int threads,
                                 uint8_t *planes, float *winrates);
 
+/* FastState::set_handicap (FastState.cpp:566-568): the handicap stones'
+ * compensation, added to komi wherever a position is scored (playouts,
+ * lz_position_score; FastState.cpp:395-401). 0 after lz_position_new. */
+int lz_position_set_handicap(lz_position *pos, int handicap);
+
 /* FastBoard::final_mc_score (FastBoard.cpp:420-443) with the position's
- * komi: black stones plus empty points whose four neighbours are black or
- * edge, minus the same for white, minus komi. */
+ * komi plus handicap (FastState::calculate_mc_score): black stones plus
+ * empty points whose four neighbours are black or edge, minus the same for
+ * white, minus komi + handicap. */
 int lz_position_score(const lz_position *pos, float *score);
 
 /* Training data of the surrogate: generator positions from seeds

@@ -588,6 +588,7 @@ void GameState::init(int side, float k) {
     passes = 0;
     movenum = 0;
     last_was_capture = false;
+    handicap = 0;
 }
 
 // FastState::play_move(color, vertex) / play_pass (FastState.cpp:345-389).

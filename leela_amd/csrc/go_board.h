@@ -147,6 +147,10 @@ struct GameState {
     int passes = 0;
     int movenum = 0;
     bool last_was_capture = false;  // the last move captured stones
+    int handicap = 0;     // FastState::m_handicap: added to komi in scores
+
+    // The komi that scores use (FastState.cpp:395-401: komi + handicap).
+    float score_komi() const { return komi + (float)handicap; }
 
     void init(int side, float k);
     int to_move() const { return board.to_move(); }

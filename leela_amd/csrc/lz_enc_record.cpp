@@ -22,7 +22,7 @@ void pack_record(const go::GameState &st, int64_t offset, EncRecord *r) {
     r->out_offset = offset;
     r->passes = st.passes;
     r->movenum = st.movenum;
-    r->komi = st.komi;
+    r->komi = st.score_komi();
     r->last_was_capture = st.last_was_capture ? 1 : 0;
 }
 

@@ -31,7 +31,7 @@ struct alignas(64) EncRecord {
     int64_t out_offset;        // byte offset of this board's planes
     int32_t passes;            // passes in a row (playouts stop at 4)
     int32_t movenum;           // moves played (playouts stop at 2 side^2)
-    float komi;
+    float komi;                // scoring komi: komi + handicap
     int32_t last_was_capture;  // the last move captured stones
     uint8_t pad_[16];
 };

@@ -177,9 +177,15 @@ int lz_make_batch(int n, int side, uint32_t seed0, int max_moves, int net,
     return LZ_OK;
 }
 
+int lz_position_set_handicap(lz_position *pos, int handicap) {
+    if (!pos || handicap < 0) return LZ_ERR_ARG;
+    pos->st.handicap = handicap;
+    return LZ_OK;
+}
+
 int lz_position_score(const lz_position *pos, float *score) {
     if (!pos || !score) return LZ_ERR_ARG;
-    *score = lz::go::final_mc_score(pos->st.board, pos->st.komi);
+    *score = lz::go::final_mc_score(pos->st.board, pos->st.score_komi());
     return LZ_OK;
 }
 

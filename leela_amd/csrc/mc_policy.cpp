@@ -175,7 +175,7 @@ void setup(PlayState &ps, const go::GameState &st) {
     ps.last_was_capture = st.last_was_capture;
     ps.passes = st.passes;
     ps.movenum = st.movenum;
-    ps.komi = st.komi;
+    ps.komi = st.score_komi();
 }
 
 float pattern_playout(const go::GameState &st, Rng &rng, const Tables &t) {

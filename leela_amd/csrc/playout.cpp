@@ -53,7 +53,7 @@ float playout(GameState &st, Rng &rng) {
         }
         st.play(move);
     }
-    return final_mc_score(b, st.komi);
+    return final_mc_score(b, st.score_komi());
 }
 
 uint32_t playout_seed(uint32_t seed, uint32_t j) {

@@ -46,6 +46,7 @@ def _lib():
         L.lz_encode_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_int,
                                       C.c_int, C.c_int, u8p]
         L.lz_position_score.argtypes = [vp, f32p]
+        L.lz_position_set_handicap.argtypes = [vp, C.c_int]
         L.lz_playout_winrate.argtypes = [vp, C.c_int, C.c_uint32, f32p]
         L.lz_make_value_labels.argtypes = [C.c_int, C.c_int, C.c_uint32,
                                            C.c_int, C.c_int, C.c_uint32,
@@ -135,6 +136,10 @@ class Position:
         _chk(self._L.lz_board_queries(self._p, a.ctypes.data_as(
             C.POINTER(C.c_int32))), "queries")
         return a
+
+    def set_handicap(self, handicap: int):
+        """FastState::set_handicap: added to komi in every score."""
+        _chk(self._L.lz_position_set_handicap(self._p, handicap), "handicap")
 
     def score(self) -> float:
         """FastBoard::final_mc_score with the position's komi."""

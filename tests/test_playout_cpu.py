@@ -125,3 +125,30 @@ def test_value_labels_match_single_entry_points():
     _, w2 = positions.value_labels(6, 100, 16, 900, side=9, max_moves=48,
                                    threads=1, planes=False)
     assert np.array_equal(w, w2)
+
+
+def test_handicap_adds_to_komi_in_every_score():
+    """FastState::calculate_mc_score scores with komi + handicap
+    (FastState.cpp:395-401): a position with handicap h scores, and plays
+    out, exactly as the same position with komi raised by h."""
+    from leela_amd.positions import PLAYOUT_LIGHT, PLAYOUT_PATTERN
+    a = Position.random(17, 9, 40)
+    # The same generator game with handicap 0 and with handicap 3.
+    base = Position.random(17, 9, 40)
+    sq = base.squares()
+    for h in (0, 3):
+        p = Position.random(17, 9, 40)
+        p.set_handicap(h)
+        q = Position.random(17, 9, 40)
+        np.testing.assert_array_equal(p.squares(), sq)
+        if h == 0:
+            s0 = p.score()
+        else:
+            assert p.score() == s0 - 3.0
+            for pol in (PLAYOUT_LIGHT, PLAYOUT_PATTERN):
+                ws = p.playout_scores(16, 5, pol)
+                w0 = q.playout_scores(16, 5, pol)
+                # The same moves: only the final subtraction differs.
+                np.testing.assert_array_equal(ws, w0 - 3.0)
+    with pytest.raises(Exception):
+        a.set_handicap(-1)
