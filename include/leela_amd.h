@@ -129,6 +129,22 @@ int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
                       const uint8_t *d_planes, float *d_raw, float *d_out,
                       void *stream);
 
+/* Caller-owned workspaces: lz_forward_device_ws is lz_forward_device with
+ * the activation buffers, value-head planes and layered tiles of `ws`
+ * instead of the context's, so forwards on different streams with
+ * different workspaces run concurrently (one workspace still serves one
+ * stream at a time: calls sharing it must be ordered by the caller). A
+ * workspace grows to the largest batch it is given (it is created for
+ * max_boards). No OpenCL counterpart: the reference keeps one set of
+ * buffers per thread (OpenCL.cpp:446-472); this is that, for device-buffer
+ * callers. lz_workspace_destroy waits for the device to be idle. */
+typedef struct lz_workspace lz_workspace;
+int lz_workspace_create(lz_ctx *ctx, int max_boards, lz_workspace **ws);
+void lz_workspace_destroy(lz_workspace *ws);
+int lz_forward_device_ws(lz_ctx *ctx, lz_workspace *ws, int net,
+                         int precision, int n, const uint8_t *d_planes,
+                         float *d_raw, float *d_out, void *stream);
+
 /* Ragged batch: n boards of sides[i] in 5..19 (e.g. mixed 9x9 / 13x13 /
  * 19x19) in one persistent launch of the 16-bit fused kernels
  * (LZ_PREC_BF16 / LZ_PREC_FP16; FP32 returns LZ_ERR_UNSUPPORTED). Planes

@@ -73,6 +73,11 @@ def load_library():
     L.lz_push_batchnorm.argtypes = [vp, C.c_int, C.c_int, f32p, f32p,
                                     C.c_float]
     L.lz_finalize.argtypes = [vp]
+    L.lz_workspace_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.lz_workspace_destroy.argtypes = [vp]
+    L.lz_workspace_destroy.restype = None
+    L.lz_forward_device_ws.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, vp,
+                                       vp, vp, vp]
     L.lz_set_softmax_temp.argtypes = [vp, C.c_float]
     L.lz_set_layered_max.argtypes = [vp, C.c_int]
     L.lz_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int,
@@ -229,6 +234,27 @@ class Backend:
         self._check(self._L.lz_forward_device(self._ctx, net, precision, n,
                                               d_planes, d_raw, d_out, stream),
                     "lz_forward_device")
+
+    def workspace(self, max_boards: int):
+        """lz_workspace_create: a workspace for forward_device_ws (free it
+        with free_workspace)."""
+        w = C.c_void_p()
+        self._check(self._L.lz_workspace_create(self._ctx, max_boards,
+                                                C.byref(w)),
+                    "lz_workspace_create")
+        return w
+
+    def free_workspace(self, w):
+        self._L.lz_workspace_destroy(w)
+
+    def forward_device_ws(self, ws, net: int, precision: int, n: int,
+                          d_planes: int, d_raw: int, d_out: int,
+                          stream: int | None = None):
+        """lz_forward_device with the caller's workspace."""
+        self._check(self._L.lz_forward_device_ws(self._ctx, ws, net,
+                                                 precision, n, d_planes,
+                                                 d_raw, d_out, stream),
+                    "lz_forward_device_ws")
 
     def forward_device_noalloc(self, net: int, precision: int, n: int,
                                d_planes: int, d_raw: int, d_out: int,

@@ -362,6 +362,74 @@ int lz_forward_device(lz_ctx *ctx, int net, int precision, int n,
     return forward_locked(ctx, net, precision, n, d_planes, d_raw, d_out, s);
 }
 
+}  // extern "C"
+
+// A caller-owned workspace (lz_workspace_create): its own activation
+// buffers, value-head planes and layered tiles, so that forwards on
+// different streams with different workspaces can overlap.
+struct lz_workspace {
+    lz_ctx *ctx = nullptr;
+    std::mutex mu;  // guards ws (growth) against concurrent calls
+    lz::Workspace ws;
+};
+
+extern "C" {
+
+int lz_workspace_create(lz_ctx *ctx, int max_boards, lz_workspace **out) {
+    if (!ctx || !out || max_boards < 1) return LZ_ERR_ARG;
+    *out = nullptr;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (!ctx->finalized)
+        return ctx->fail(LZ_ERR_STATE, "workspace before lz_finalize");
+    (void)hipSetDevice(ctx->device);
+    lz_workspace *w = new (std::nothrow) lz_workspace();
+    if (!w) return LZ_ERR_NOMEM;
+    w->ctx = ctx;
+    std::string msg;
+    const int rc = lz::ensure_workspace(w->ws, ctx->net, max_boards, &msg);
+    if (rc != LZ_OK) {
+        lz::free_workspace(w->ws);
+        delete w;
+        return ctx->fail(rc, "%s", msg.c_str());
+    }
+    *out = w;
+    return LZ_OK;
+}
+
+void lz_workspace_destroy(lz_workspace *w) {
+    if (!w) return;
+    (void)hipSetDevice(w->ctx->device);
+    (void)hipDeviceSynchronize();  // no launch may still use the buffers
+    lz::free_workspace(w->ws);
+    delete w;
+}
+
+int lz_forward_device_ws(lz_ctx *ctx, lz_workspace *w, int net,
+                         int precision, int n, const uint8_t *d_planes,
+                         float *d_raw, float *d_out, void *stream) {
+    if (!ctx || !w || w->ctx != ctx) return LZ_ERR_ARG;
+    float temperature;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (!ctx->finalized)
+            return ctx->fail(LZ_ERR_STATE, "forward before lz_finalize");
+        temperature = ctx->temperature;
+    }
+    if (net != LZ_NET_POLICY && net != LZ_NET_VALUE) return LZ_ERR_ARG;
+    if (precision < LZ_PREC_FP32 || precision > LZ_PREC_FP16) return LZ_ERR_ARG;
+    if (n < 0 || (n > 0 && !d_planes)) return LZ_ERR_ARG;
+    if (n == 0) return LZ_OK;
+    (void)hipSetDevice(ctx->device);
+    std::lock_guard<std::mutex> g(w->mu);
+    std::string msg;
+    int rc = lz::ensure_workspace(w->ws, ctx->net, n, &msg);
+    if (rc == LZ_OK)
+        rc = lz::run_net(ctx->net[net], net, precision, n, d_planes, d_raw,
+                         d_out, temperature, w->ws,
+                         static_cast<hipStream_t>(stream), &msg);
+    return rc;
+}
+
 int lz_forward(lz_ctx *ctx, int net, int precision, int n,
                const uint8_t *planes, float *raw, float *out) {
     if (!ctx) return LZ_ERR_ARG;
