@@ -508,12 +508,27 @@ def main() -> int:
     pdenc.playouts_into(po_pos, po_k, 5, d_po.data_ptr(), stream)
     torch.cuda.synchronize(dev)
     gpu_po_s = po_n * po_k / (time.perf_counter() - t0)
+    # The reference's pattern policy on the device (one playout per thread;
+    # the first call uploads its tables and allocates the playout states).
+    d_pw = torch.empty(po_n, dtype=torch.float32, device=dev)
+    pdenc.playouts_into(po_pos, 4, 6, d_pw.data_ptr(), stream,
+                        positions.PLAYOUT_PATTERN)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    pdenc.playouts_into(po_pos, po_k, 6, d_pw.data_ptr(), stream,
+                        positions.PLAYOUT_PATTERN)
+    torch.cuda.synchronize(dev)
+    gpu_pat_s = po_n * po_k / (time.perf_counter() - t0)
     pdenc.close()
     host_n = 128
     t0 = time.perf_counter()
     positions.value_labels(host_n, 10000, po_k, 5, threads=enc_threads,
                            planes=False)
     host_po_s = host_n * po_k / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    positions.value_labels(host_n, 10000, po_k, 6, threads=enc_threads,
+                           planes=False, policy=positions.PLAYOUT_PATTERN)
+    host_pat_s = host_n * po_k / (time.perf_counter() - t0)
     denc.close()
 
     # Network::benchmark restated (the netbench command): 16 threads of
@@ -686,6 +701,11 @@ def main() -> int:
                 "sample": f"{po_n} generator positions x {po_k} light "
                           f"playouts on the device, {host_n} x {po_k} on "
                           f"the host; equal results bit for bit",
+                "device_pattern_playouts_per_s": round(gpu_pat_s, 1),
+                "host_pattern_playouts_per_s": round(host_pat_s, 1),
+                "pattern": "the reference's playout policy "
+                           "(FastState::play_random_move), same positions "
+                           "and counts; device equal to host bit for bit",
             },
         },
         "roofline": {
