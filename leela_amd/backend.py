@@ -73,6 +73,9 @@ def load_library():
     L.lz_push_batchnorm.argtypes = [vp, C.c_int, C.c_int, f32p, f32p,
                                     C.c_float]
     L.lz_finalize.argtypes = [vp]
+    L.lz_network_initialize.argtypes = [vp]
+    L.lz_network_get.argtypes = []
+    L.lz_network_get.restype = vp
     L.lz_workspace_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
     L.lz_workspace_destroy.argtypes = [vp]
     L.lz_workspace_destroy.restype = None
@@ -153,9 +156,24 @@ class Backend:
     def __init__(self, device: int = 0):
         self._L = load_library()
         self._ctx = C.c_void_p()
+        self._shared = False
         rc = self._L.lz_init(C.byref(self._ctx), device)
         if rc != LZ_OK:
             raise LeelaError(rc, self._L.lz_last_error(None).decode())
+
+    @classmethod
+    def default(cls) -> "Backend":
+        """Network::get_Network: the process-wide context with the
+        reference's layer tables and the synthetic weight set
+        (lz_network_get); close() leaves it alive."""
+        b = cls.__new__(cls)
+        b._L = load_library()
+        ptr = b._L.lz_network_get()
+        if not ptr:
+            raise LeelaError(LZ_ERR_STATE, "lz_network_get failed")
+        b._ctx = C.c_void_p(ptr)
+        b._shared = True
+        return b
 
     def _check(self, rc: int, what: str):
         if rc != LZ_OK:
@@ -393,9 +411,9 @@ class Backend:
         self._check(self._L.lz_queue_stop(self._ctx), "lz_queue_stop")
 
     def close(self):
-        if self._ctx:
+        if self._ctx and not self._shared:
             self._L.lz_destroy(self._ctx)
-            self._ctx = C.c_void_p()
+        self._ctx = C.c_void_p()
 
     def __del__(self):
         try:

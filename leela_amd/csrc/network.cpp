@@ -396,6 +396,61 @@ int lz_network_heatmap(lz_ctx *ctx, int precision, const lz_position *pos,
     return LZ_OK;
 }
 
+int lz_network_initialize(lz_ctx *ctx) {
+    if (!ctx) return LZ_ERR_ARG;
+    // Network::initialize (Network.cpp:201-233): the policy net's 13 convs,
+    // the value net's 12 convs and two inner products, in push order.
+    struct Shape {
+        int kind, filter, cin, cout;
+    };
+    std::vector<Shape> shapes[2];
+    shapes[0].push_back({0, 5, 32, 96});
+    shapes[0].push_back({0, 3, 96, 128});
+    for (int i = 0; i < 10; i++) shapes[0].push_back({0, 3, 128, 128});
+    shapes[0].push_back({0, 3, 128, 1});
+    shapes[1].push_back({0, 5, 32, 64});
+    for (int i = 0; i < 10; i++) shapes[1].push_back({0, 3, 64, 64});
+    shapes[1].push_back({0, 3, 64, 1});
+    shapes[1].push_back({1, 0, 361, 256});
+    shapes[1].push_back({1, 0, 256, 1});
+    // The reference compiles its weights in (NN128.cpp, NNValue.cpp, not
+    // shipped); these are He-normal, N(0, 2 / fan_in), layer l of net k
+    // from the reference RNG stream 7000 + 100 k + l, biases zero.
+    for (int k = 0; k < 2; k++) {
+        for (size_t l = 0; l < shapes[k].size(); l++) {
+            const Shape &s = shapes[k][l];
+            const int fan_in = s.kind == 0 ? s.cin * s.filter * s.filter
+                                           : s.cin;
+            std::vector<float> w((size_t)s.cout * fan_in), b(s.cout, 0.0f);
+            lz::normal_fill(7000 + 100 * k + (uint32_t)l, w.size(),
+                            (float)std::sqrt(2.0 / fan_in), w.data());
+            const int rc =
+                s.kind == 0
+                    ? lz_push_convolve(ctx, k, s.filter, s.cin, s.cout,
+                                       w.data(), b.data())
+                    : lz_push_innerproduct(ctx, k, s.cin, s.cout, w.data(),
+                                           b.data());
+            if (rc != LZ_OK) return rc;
+        }
+    }
+    return lz_finalize(ctx);
+}
+
+lz_ctx *lz_network_get(void) {
+    static std::once_flag once;
+    static lz_ctx *net = nullptr;
+    std::call_once(once, [] {
+        lz_ctx *c = nullptr;
+        if (lz_init(&c, 0) != LZ_OK) return;
+        if (lz_network_initialize(c) != LZ_OK) {
+            lz_destroy(c);
+            return;
+        }
+        net = c;
+    });
+    return net;
+}
+
 int lz_network_benchmark(lz_ctx *ctx, int precision, const lz_position *pos,
                          int threads, float *policy_ps, float *value_ps) {
     if (!ctx || !pos || !policy_ps || !value_ps || threads < 1)

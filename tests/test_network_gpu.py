@@ -294,3 +294,34 @@ def test_show_heatmap_text(be):
         assert ln == f"{p:1.3f} ({col}{y + 1})"
         cum += p
     assert len(top) >= 1
+
+
+def test_network_get_is_the_reference_net_with_set_a(weights_a):
+    """Network::get_Network (lz_network_get): one process-wide context,
+    initialised once with the reference's layer tables and weight set A;
+    its forwards equal a context loaded with set A from Python, bit for
+    bit, and closing a handle to it leaves it alive."""
+    import leela_amd
+    from leela_amd.positions import make_batch
+    d1 = leela_amd.Backend.default()
+    d2 = leela_amd.Backend.default()
+    assert d1._ctx.value == d2._ctx.value
+    ref = leela_amd.Backend(0)
+    pol, val = weights_a
+    ref.load_net(leela_amd.NET_POLICY, pol)
+    ref.load_net(leela_amd.NET_VALUE, val)
+    ref.finalize()
+    try:
+        for net in (NET_POLICY, NET_VALUE):
+            planes = make_batch(9, seed0=4242 + net, net=net)
+            for prec in (leela_amd.PREC_FP32, leela_amd.PREC_FP16):
+                got = d1.forward(net, prec, planes)
+                want = ref.forward(net, prec, planes)
+                np.testing.assert_array_equal(got[0], want[0])
+                np.testing.assert_array_equal(got[1], want[1])
+        d1.close()
+        raw, _ = d2.forward(NET_POLICY, leela_amd.PREC_FP16,
+                            make_batch(2, seed0=1))
+        assert np.isfinite(raw).all()
+    finally:
+        ref.close()
