@@ -225,6 +225,64 @@ __device__ __forceinline__ void conv_tile(const T *act,
     }
 }
 
+// conv_tile for the layered kernels, whose waves issue few MFMAs per
+// k-step (one wave per SIMD, 10 MFMAs per step in a trunk band): fully
+// unrolled, the A fragments prefetched PF k-steps ahead so that more of
+// their L2 latency is covered, the B fragments one step ahead. Every
+// accumulator sees the same MFMAs in the same order as in conv_tile.
+template <typename T, int F, int CIN, int MT, int MTW, int PS, int NT, int PF>
+__device__ __forceinline__ void conv_tile_deep(
+    const T *act, const typename Op<T>::v8 *__restrict__ wpk,
+    f32x4 (&acc)[MTW][NT], const int (&pb)[NT], int rp, int mt0, int lane,
+    const float *__restrict__ bias = nullptr) {
+    typedef typename Op<T>::v8 v8;
+    constexpr int CB = CIN / 32;
+    constexpr int STEPS = F * F * CB;
+    const int g8 = (lane >> 4) * 8;
+#pragma unroll
+    for (int m = 0; m < MTW; m++) {
+        f32x4 init = {0.f, 0.f, 0.f, 0.f};
+        if (bias) {
+            const int o0 = (mt0 + m) * 16 + (lane >> 4) * 4;
+#pragma unroll
+            for (int r = 0; r < 4; r++) init[r] = bias[o0 + r];
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[m][t] = init;
+    }
+    const v8 *wl = wpk + mt0 * 64 + lane;
+    auto b_off = [&](int s) {
+        const int tap = s / CB, cb = s % CB;
+        return (tap / F) * rp + (tap % F) * PS + cb * 32 + g8;
+    };
+    v8 a[PF][MTW], b[2][NT];
+#pragma unroll
+    for (int p = 0; p < PF; p++)
+        if (p < STEPS) {
+#pragma unroll
+            for (int m = 0; m < MTW; m++) a[p][m] = wl[(p * MT + m) * 64];
+        }
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+        b[0][t] = *reinterpret_cast<const v8 *>(act + pb[t] + b_off(0));
+#pragma unroll
+    for (int s = 0; s < STEPS; s++) {
+        if (s + 1 < STEPS) {
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                b[(s + 1) & 1][t] =
+                    *reinterpret_cast<const v8 *>(act + pb[t] + b_off(s + 1));
+        }
+#pragma unroll
+        for (int m = 0; m < MTW; m++) {
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                acc[m][t] = Op<T>::mfma(a[s % PF][m], b[s & 1][t], acc[m][t]);
+            if (s + PF < STEPS) a[s % PF][m] = wl[((s + PF) * MT + m) * 64];
+        }
+    }
+}
+
 // ELU of acc (whose bias the conv folded into the accumulator start),
 // written into the trunk tile at the lane's pixel offsets po[t] (-1:
 // padding column). D layout of 16x16 MFMA: lane holds rows 4*(lane/16)+r
@@ -774,6 +832,11 @@ hipError_t launch(int which, int wm, const uint8_t *planes,
 // bit.
 constexpr int kLS = 19;
 constexpr int kLMT = 2;  // output m-tiles per workgroup
+// A prefetch distance of the layered convs, in k-steps. Measured at 1 / 16
+// / 64 boards (policy, FP16): 2 steps 97 / 113 / 155 us per forward; 1, 3,
+// 4 and 6 steps 118-122 / 141 / 186 (the value net 93 / 100 / 117 for
+// every distance).
+constexpr int kLayeredPF = 2;
 template <int PS>
 constexpr int lrp() { return (kLS + 2) * PS + row_pad(PS, 2); }
 constexpr int lrp_in() { return (kLS + 4) * kInPS + row_pad(kInPS, 4); }
@@ -874,9 +937,9 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
     }
     __syncthreads();
     f32x4 acc[kLMT][kNT];
-    conv_tile<T, 5, 32, MT1, kLMT, kInPS>(s_act, static_cast<const v8 *>(w),
-                                          acc, pb_in, rp_in, cg * kLMT, lane,
-                                          bias);
+    conv_tile_deep<T, 5, 32, MT1, kLMT, kInPS, kNT, kLayeredPF>(
+        s_act, static_cast<const v8 *>(w), acc, pb_in, rp_in, cg * kLMT, lane,
+        bias);
     store_tile<T, kLMT>(out_tiles + (size_t)board * ltile<PS>(), acc,
                         cg * kLMT, po, lane);
 }
@@ -956,7 +1019,7 @@ __global__ __launch_bounds__(kWN * 64) void layered_conv(
     wait_lds_copies();
     __syncthreads();
     f32x4 acc[MTW][kBandNT];
-    conv_tile<T, 3, CIN, MT, MTW, PS, kBandNT>(
+    conv_tile_deep<T, 3, CIN, MT, MTW, PS, kBandNT, kLayeredPF>(
         s_act, static_cast<const v8 *>(w), acc, pb, lrp<PS>(), wave * MTW,
         lane, bias);
     store_tile<T, MTW, kBandNT>(out_tiles + (size_t)board * TILE, acc,
@@ -984,8 +1047,8 @@ __global__ __launch_bounds__(kWN * 64) void layered_head(
     __syncthreads();
     if (wave != 0) return;
     f32x4 acc[1][kBandNT];
-    conv_tile<T, 3, C, 1, 1, PS, kBandNT>(s_act, static_cast<const v8 *>(w),
-                                          acc, pb, lrp<PS>(), 0, lane);
+    conv_tile_deep<T, 3, C, 1, 1, PS, kBandNT, kLayeredPF>(
+        s_act, static_cast<const v8 *>(w), acc, pb, lrp<PS>(), 0, lane);
     const float hb = hbias[0];
     if (lane < 16) {
 #pragma unroll

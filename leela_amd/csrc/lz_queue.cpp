@@ -102,6 +102,9 @@ struct Batch {
 // How long the dispatcher polls for a batch that is filling before it
 // sleeps on its condition variable.
 constexpr auto kDispatchPoll = std::chrono::microseconds(500);
+// How long the completion thread polls a launched batch's event before it
+// blocks on it.
+constexpr auto kCompletePoll = std::chrono::microseconds(1000);
 
 class LeafQueue {
 public:
@@ -236,15 +239,19 @@ private:
         hipError_t e = hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking);
         if (e == hipSuccess)
             e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+        // The rows and results are coherent host memory: the layered
+        // kernels of small batches read and write them in place.
         if (e == hipSuccess)
             e = hipHostMalloc(reinterpret_cast<void **>(&b.h_planes),
-                              (size_t)cap_ * kPos, hipHostMallocDefault);
+                              (size_t)cap_ * kPos, hipHostMallocCoherent);
         if (e == hipSuccess)
             e = hipHostMalloc(reinterpret_cast<void **>(&b.h_raw),
-                              sizeof(float) * cap_ * kOut, hipHostMallocDefault);
+                              sizeof(float) * cap_ * kOut,
+                              hipHostMallocCoherent);
         if (e == hipSuccess)
             e = hipHostMalloc(reinterpret_cast<void **>(&b.h_out),
-                              sizeof(float) * cap_ * kOut, hipHostMallocDefault);
+                              sizeof(float) * cap_ * kOut,
+                              hipHostMallocCoherent);
         if (e == hipSuccess)
             e = hipHostMalloc(reinterpret_cast<void **>(&b.h_table),
                               table_bytes(cap_), hipHostMallocDefault);
@@ -490,38 +497,57 @@ private:
         const size_t tbytes =
             kTableHead + (ragged ? sizeof(FusedSlot) * ns : 0);
         const size_t per = b.net == LZ_NET_POLICY ? kOut : 1;
-        hipError_t e = hipMemcpyAsync(b.d_table, b.h_table, tbytes,
-                                      hipMemcpyHostToDevice, b.stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b.d_planes, b.h_planes, (size_t)n * kPos,
-                               hipMemcpyHostToDevice, b.stream);
-        // Small dense batches run the layered kernels, launched directly:
-        // a graph of their ~14 nodes costs more host time to launch
-        // (~7 us a node) than the launches themselves, which overlap the
-        // kernels already running.
         const Net &net = ctx_->net[b.net];
-        if (e == hipSuccess && !ragged && n <= net.layered_max &&
-            b.ws.tiles) {
+        hipError_t e = hipSuccess;
+        if (!ragged && n <= net.layered_max && b.ws.tiles) {
+            // Small dense batches: the layered kernels, launched directly
+            // (a graph of their ~14 nodes costs more host time to launch,
+            // ~7 us a node, than the launches, which overlap the kernels
+            // already running), reading the rows from and writing the
+            // results into the staging buffers in host memory. No copies:
+            // on either side of a small batch a copy's latency is that of
+            // several of its convs. (The layered kernels take the batch
+            // size as an argument, not from the batch table.)
             std::string msg;
-            if (run_net(net, b.net, precision_, n, b.d_planes, b.d_raw,
-                        b.d_out, ctx_->temperature, b.ws, b.stream,
+            if (run_net(net, b.net, precision_, n, b.h_planes, b.h_raw,
+                        b.h_out, ctx_->temperature, b.ws, b.stream,
                         &msg) != LZ_OK)
                 e = hipErrorLaunchFailure;
-        } else if (e == hipSuccess) {
-            e = hipGraphLaunch(b.graph[b.net][k][ragged ? 1 : 0], b.stream);
+        } else {
+            e = hipMemcpyAsync(b.d_table, b.h_table, tbytes,
+                               hipMemcpyHostToDevice, b.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b.d_planes, b.h_planes, (size_t)n * kPos,
+                                   hipMemcpyHostToDevice, b.stream);
+            if (e == hipSuccess)
+                e = hipGraphLaunch(b.graph[b.net][k][ragged ? 1 : 0],
+                                   b.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b.h_raw, b.d_raw, sizeof(float) * per * n,
+                                   hipMemcpyDeviceToHost, b.stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b.h_out, b.d_out, sizeof(float) * per * n,
+                                   hipMemcpyDeviceToHost, b.stream);
         }
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b.h_raw, b.d_raw, sizeof(float) * per * n,
-                               hipMemcpyDeviceToHost, b.stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b.h_out, b.d_out, sizeof(float) * per * n,
-                               hipMemcpyDeviceToHost, b.stream);
         if (e == hipSuccess) e = hipEventRecord(b.done, b.stream);
         b.status = e == hipSuccess ? LZ_OK : LZ_ERR_HIP;
         std::lock_guard<std::mutex> g(mu_);
         launches_++;
         positions_ += n;
         per_bucket_[k]++;
+    }
+
+    // A batch's event: polled (yielding) for up to kCompletePoll, the
+    // length of a small batch, then waited on; a blocking wait's wake-up
+    // alone costs tens of microseconds.
+    static hipError_t wait_done(hipEvent_t ev) {
+        const auto until = Clock::now() + kCompletePoll;
+        for (;;) {
+            const hipError_t q = hipEventQuery(ev);
+            if (q != hipErrorNotReady) return q;
+            if (Clock::now() >= until) return hipEventSynchronize(ev);
+            std::this_thread::yield();
+        }
     }
 
     // Completion thread: batches in launch order; callbacks, then the
@@ -540,7 +566,7 @@ private:
                 inflight_.pop_front();
             }
             int status = b->status;
-            if (status == LZ_OK && hipEventSynchronize(b->done) != hipSuccess)
+            if (status == LZ_OK && wait_done(b->done) != hipSuccess)
                 status = LZ_ERR_HIP;
             const size_t per = b->net == LZ_NET_POLICY ? kOut : 1;
             t_in_callback = true;
