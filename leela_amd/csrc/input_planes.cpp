@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace lz {
 namespace {
@@ -112,12 +113,45 @@ int rev_rotate_nn_idx(int idx, int symmetry, int side) {
     return rotate_nn_idx(idx, inverse[symmetry], side);
 }
 
+namespace {
+// rotate_nn_idx as tables, per side (1..19) and symmetry, built once: the
+// per-element index arithmetic dominated a whole host encode.
+constexpr int kTableSides = 19;
+struct RotationTables {
+    std::vector<uint16_t> idx[kTableSides + 1][8];
+    RotationTables() {
+        for (int side = 1; side <= kTableSides; side++)
+            for (int sym = 0; sym < 8; sym++) {
+                auto &t = idx[side][sym];
+                t.resize((size_t)side * side);
+                for (int i = 0; i < side * side; i++)
+                    t[i] = (uint16_t)rotate_nn_idx(i, sym, side);
+            }
+    }
+};
+}  // namespace
+
 void rotate_planes(const uint8_t *planes, int symmetry, int side,
                    uint8_t *out) {
     const int P = side * side;
-    for (int c = 0; c < kPlanes; c++)
-        for (int i = 0; i < P; i++)
-            out[c * P + i] = planes[c * P + rotate_nn_idx(i, symmetry, side)];
+    if (symmetry == 0) {
+        std::memcpy(out, planes, (size_t)kPlanes * P);
+        return;
+    }
+    if (side < 1 || side > kTableSides) {
+        for (int c = 0; c < kPlanes; c++)
+            for (int i = 0; i < P; i++)
+                out[c * P + i] =
+                    planes[c * P + rotate_nn_idx(i, symmetry, side)];
+        return;
+    }
+    static const RotationTables tables;
+    const uint16_t *t = tables.idx[side][symmetry].data();
+    for (int c = 0; c < kPlanes; c++) {
+        const uint8_t *src = planes + c * P;
+        uint8_t *dst = out + c * P;
+        for (int i = 0; i < P; i++) dst[i] = src[t[i]];
+    }
 }
 
 }  // namespace lz

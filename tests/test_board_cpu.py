@@ -231,6 +231,17 @@ def test_symmetry_planes_rotate():
         assert np.array_equal(back, base)
 
 
+@pytest.mark.parametrize("side", [5, 9, 13])
+def test_symmetry_planes_rotate_small_boards(side):
+    p = Position.random(12, side, 60)
+    for net in (NET_POLICY, NET_VALUE):
+        base = p.planes(net, 0)
+        for s in range(8):
+            idx = np.array([rot_closed_form(i, s, side)
+                            for i in range(side * side)])
+            assert np.array_equal(p.planes(net, s), base[:, idx])
+
+
 def test_make_batch_matches_single_positions():
     b = positions.make_batch(6, seed0=40, net=NET_VALUE, threads=3)
     for i in range(6):
