@@ -894,11 +894,12 @@ __device__ __forceinline__ int layered_boards(int n, const int *counts) {
 }
 
 // Stem: 5x5, 32 planes -> M1, written into the trunk tile of the board.
+// bits (optional): the planes packed one word per point, read instead.
 template <typename T, int PS, int M1>
 __global__ __launch_bounds__(kWN * 64) void layered_stem(
-    const uint8_t *__restrict__ planes, const void *w,
-    const float *__restrict__ bias, T *__restrict__ out_tiles, int n,
-    const int *counts) {
+    const uint8_t *__restrict__ planes, const uint32_t *__restrict__ bits,
+    const void *w, const float *__restrict__ bias, T *__restrict__ out_tiles,
+    int n, const int *counts) {
     typedef typename Op<T>::v8 v8;
     constexpr int MT1 = M1 / 16, G = MT1 / kLMT;
     constexpr int S = kLS, pin = S + 4, rp_in = lrp_in();
@@ -911,12 +912,15 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
     // The board's planes into LDS: whole KiB chunks directly, the 288-byte
     // tail through registers (no read past the board).
     const uint8_t *src = planes + (size_t)board * kLPlaneBytes;
+    const uint32_t *wb = bits ? bits + (size_t)board * kP : nullptr;
     constexpr int kFull = kLPlaneBytes / 1024, kTail = kLPlaneBytes % 1024;
     static_assert(kTail % 16 == 0 && kLPlaneBytes % 16 == 0, "plane copy");
-    copy_to_lds(src, s_pl, kFull, wave, lane);
-    if (tid < kTail / 16)
-        reinterpret_cast<uint4 *>(s_pl + kFull * 1024)[tid] =
-            reinterpret_cast<const uint4 *>(src + kFull * 1024)[tid];
+    if (!wb) {
+        copy_to_lds(src, s_pl, kFull, wave, lane);
+        if (tid < kTail / 16)
+            reinterpret_cast<uint4 *>(s_pl + kFull * 1024)[tid] =
+                reinterpret_cast<const uint4 *>(src + kFull * 1024)[tid];
+    }
     int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
     lgeom<PS>(wave * kNT, lane, pb_in, pb, po, pix);
     wait_lds_copies();
@@ -929,9 +933,16 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
         const int row = pp / pin - 2, xx = pp % pin - 2;
         const bool in = row >= 0 && row < S && xx >= 0 && xx < S;
         v8 v;
+        if (wb) {
+            const uint32_t word = in ? wb[row * S + xx] >> cg8 : 0u;
 #pragma unroll
-        for (int e = 0; e < 8; e++)
-            v[e] = (T)(in ? (float)s_pl[(cg8 + e) * kP + row * S + xx] : 0.0f);
+            for (int e = 0; e < 8; e++) v[e] = (T)(float)((word >> e) & 1u);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                v[e] = (T)(in ? (float)s_pl[(cg8 + e) * kP + row * S + xx]
+                              : 0.0f);
+        }
         *reinterpret_cast<v8 *>(s_act + (row + 2) * rp_in + (xx + 2) * kInPS +
                                 cg8) = v;
     }
@@ -1058,22 +1069,50 @@ __global__ __launch_bounds__(kWN * 64) void layered_head(
     }
 }
 
-// softmax(T) of each board's head plane (Network.cpp:450-469), one wave
-// per board.
-__global__ __launch_bounds__(64) void layered_softmax(
-    const float *__restrict__ plane, float *__restrict__ raw,
+// The policy head of one board, whole: the 1-channel conv (FP32
+// accumulation, ELU) over the board's tile staged in full, by the four
+// waves as in fused_net's head (six column tiles each), then softmax(T) by
+// wave 0 (Network.cpp:450-469), raw logits and probabilities straight to
+// the outputs. One launch in place of a banded head and a softmax.
+template <typename T, int PS, int C>
+__global__ __launch_bounds__(kWN * 64) void layered_policy_out(
+    const T *__restrict__ in_tiles, const void *w,
+    const float *__restrict__ hbias, float *__restrict__ raw,
     float *__restrict__ out, int n, const int *counts, float temperature) {
+    typedef typename Op<T>::v8 v8;
+    constexpr int TILE = ltile<PS>();
     const int board = blockIdx.x;
     if (board >= layered_boards(n, counts)) return;
-    const size_t o = (size_t)board * kP;
-    wave_softmax(plane + o, kP, temperature, raw ? raw + o : nullptr,
-                 out ? out + o : nullptr, threadIdx.x);
+    __shared__ __attribute__((aligned(16))) T s_act[TILE];
+    __shared__ float s_vec[kP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    copy_to_lds(in_tiles + (size_t)board * TILE, s_act,
+                TILE * (int)sizeof(T) / 1024, wave, lane);
+    int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
+    lgeom<PS>(wave * kNT, lane, pb_in, pb, po, pix);
+    wait_lds_copies();
+    __syncthreads();
+    f32x4 acc[1][kNT];
+    conv_tile_deep<T, 3, C, 1, 1, PS, kNT, kLayeredPF>(
+        s_act, static_cast<const v8 *>(w), acc, pb, lrp<PS>(), 0, lane);
+    const float hb = hbias[0];
+    if (lane < 16) {
+#pragma unroll
+        for (int t = 0; t < kNT; t++)
+            if (pix[t] >= 0) s_vec[pix[t]] = elu(acc[0][t][0] + hb);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const size_t o = (size_t)board * kP;
+        wave_softmax(s_vec, kP, temperature, raw ? raw + o : nullptr,
+                     out ? out + o : nullptr, lane);
+    }
 }
 
 template <typename T>
 hipError_t layered_run(int which, const uint8_t *planes, const FusedArgs &a,
                        float *raw, float *out, int n, float temperature,
-                       void *tiles, hipStream_t s) {
+                       void *tiles, hipStream_t s, const uint32_t *bits) {
     const dim3 blk(kWN * 64);
     // Two policy tiles per board, then two value tiles: each net keeps its
     // own layout's zero halos.
@@ -1093,22 +1132,21 @@ hipError_t layered_run(int which, const uint8_t *planes, const FusedArgs &a,
         if (a.nconv != 13 || a.c != 128 || a.m1 != 96)
             return hipErrorInvalidValue;
         hipLaunchKernelGGL((layered_stem<T, 144, 96>), dim3(n * 3), blk, 0, s,
-                           planes, a.w[0], a.b[0], buf[0], n, cnt);
+                           planes, bits, a.w[0], a.b[0], buf[0], n, cnt);
         hipLaunchKernelGGL((layered_conv<T, 144, 96, 128>), dim3(n * kBands),
                            blk, 0, s, buf[0], a.w[1], a.b[1], buf[1], n, cnt);
         for (int l = 2; l <= 11; l++)
             hipLaunchKernelGGL((layered_conv<T, 144, 128, 128>),
                                dim3(n * kBands), blk, 0, s, buf[(l + 1) & 1],
                                a.w[l], a.b[l], buf[l & 1], n, cnt);
-        hipLaunchKernelGGL((layered_head<T, 144, 128>), dim3(n * kBands), blk,
-                           0, s, buf[1], a.w[12], a.b[12], a.h0, n, cnt);
-        hipLaunchKernelGGL(layered_softmax, dim3(n), dim3(64), 0, s, a.h0,
-                           raw, out, n, cnt, temperature);
+        hipLaunchKernelGGL((layered_policy_out<T, 144, 128>), dim3(n), blk, 0,
+                           s, buf[1], a.w[12], a.b[12], raw, out, n, cnt,
+                           temperature);
         return hipGetLastError();
     }
     if (a.nconv != 12 || a.c != 64 || a.m1 != 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL((layered_stem<T, 80, 64>), dim3(n * 2), blk, 0, s,
-                       planes, a.w[0], a.b[0], buf[0], n, cnt);
+                       planes, bits, a.w[0], a.b[0], buf[0], n, cnt);
     for (int l = 1; l <= 10; l++)
         hipLaunchKernelGGL((layered_conv<T, 80, 64, 64>), dim3(n * kBands),
                            blk, 0, s, buf[(l + 1) & 1], a.w[l], a.b[l],
@@ -1154,16 +1192,16 @@ hipError_t fused_forward_layered(int which, FusedType type,
                                  const uint8_t *planes, const FusedArgs &args,
                                  float *raw, float *out, int n,
                                  float temperature, void *tiles,
-                                 hipStream_t stream) {
+                                 hipStream_t stream, const uint32_t *bits) {
     if (n <= 0) return hipSuccess;
     if (n > kLayeredMaxBoards || !tiles || args.slots ||
         type == FusedType::kFP8)
         return hipErrorInvalidValue;
     return type == FusedType::kBF16
                ? layered_run<__bf16>(which, planes, args, raw, out, n,
-                                     temperature, tiles, stream)
+                                     temperature, tiles, stream, bits)
                : layered_run<_Float16>(which, planes, args, raw, out, n,
-                                       temperature, tiles, stream);
+                                       temperature, tiles, stream, bits);
 }
 
 hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,

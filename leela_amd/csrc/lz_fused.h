@@ -86,10 +86,14 @@ hipError_t fused_forward(int which, FusedType type, const uint8_t *planes,
 // latency for small batches.
 constexpr int kLayeredMaxBoards = 64;
 size_t layered_tile_bytes();
+// bits (optional): the same boards' planes packed one 32-bit word per
+// point, bit c = plane c ([n][361]), which the stem reads instead of the
+// 32 x 361 plane bytes (planes of 0 / 1 only).
 hipError_t fused_forward_layered(int which, FusedType type,
                                  const uint8_t *planes, const FusedArgs &args,
                                  float *raw, float *out, int n,
                                  float temperature, void *tiles,
-                                 hipStream_t stream);
+                                 hipStream_t stream,
+                                 const uint32_t *bits = nullptr);
 
 }  // namespace lz

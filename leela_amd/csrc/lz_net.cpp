@@ -435,7 +435,8 @@ int run_net(const Net &net, int which, int precision, int n,
             const uint8_t *d_planes, float *d_raw, float *d_out,
             float temperature, Workspace &ws, hipStream_t stream,
             std::string *msg, LaunchTimer *timer,
-            const RaggedLayout *ragged, const int *counts) {
+            const RaggedLayout *ragged, const int *counts,
+            const uint32_t *bits) {
     if (ragged && precision != LZ_PREC_BF16 && precision != LZ_PREC_FP16) {
         *msg = "ragged batches run on the 16-bit fused kernels only";
         return LZ_ERR_UNSUPPORTED;
@@ -491,7 +492,7 @@ int run_net(const Net &net, int which, int precision, int n,
                 reinterpret_cast<uintptr_t>(d_planes) % 16 == 0)
                 e = fused_forward_layered(which, ft, d_planes, args, d_raw,
                                           d_out, n, temperature, ws.tiles,
-                                          stream);
+                                          stream, bits);
             else
                 e = fused_forward(which, ft, d_planes, args, d_raw, d_out, n,
                                   temperature, cfg, stream);

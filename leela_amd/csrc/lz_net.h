@@ -121,7 +121,7 @@ int run_net(const Net &net, int which, int precision, int n,
             float temperature, Workspace &ws, hipStream_t stream,
             std::string *msg, LaunchTimer *timer = nullptr,
             const RaggedLayout *ragged = nullptr,
-            const int *counts = nullptr);
+            const int *counts = nullptr, const uint32_t *bits = nullptr);
 // counts: device-side {slots, boards} the fused forms read when they start
 // (graph replays of a bucket on a smaller batch); n (and the ragged
 // layout's nslots) are then capacities. The FP32 form ignores it and

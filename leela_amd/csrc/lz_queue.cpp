@@ -49,6 +49,7 @@ namespace lz {
 namespace {
 constexpr int kPos = 32 * 361;  // plane bytes of a row, any side
 constexpr int kOut = 361;       // result floats of a policy row
+constexpr int kP = 361;         // points of a 19x19 board
 constexpr int kBuckets[] = {64, 256, 1024, 2048};
 constexpr int kNumBuckets = 4;
 constexpr int kSets = 4;
@@ -66,7 +67,25 @@ struct Row {
     lz_done_fn done;
     void *user;
     int side;
+    bool packed;  // 19x19 row whose planes are 0 / 1, packed into h_bits
 };
+
+// A 19x19 row's planes packed one word per point (bit c = plane c), as
+// the layered stem reads them; false if a plane byte is not 0 or 1 (the
+// row then goes in as bytes).
+bool pack_row(const uint8_t *planes, uint32_t *bits) {
+    uint32_t w[kP] = {};
+    uint8_t other = 0;
+    for (int c = 0; c < 32; c++) {
+        const uint8_t *p = planes + c * kP;
+        for (int i = 0; i < kP; i++) {
+            w[i] |= (uint32_t)(p[i] & 1u) << c;
+            other |= (uint8_t)(p[i] & ~1u);
+        }
+    }
+    std::memcpy(bits, w, sizeof(w));
+    return other == 0;
+}
 
 struct Batch {
     enum State { kFree, kOpen, kClosed, kLaunched };
@@ -83,6 +102,7 @@ struct Batch {
     float *h_raw = nullptr, *h_out = nullptr, *d_raw = nullptr,
           *d_out = nullptr;
     uint8_t *h_table = nullptr, *d_table = nullptr;
+    uint32_t *h_bits = nullptr;  // packed 19x19 rows (coherent host)
     int *d_in_off = nullptr, *d_out_off = nullptr;
     Workspace ws;
     hipStream_t stream = nullptr;
@@ -160,7 +180,7 @@ public:
             free_cv_.wait(lk);
         }
         const int row = b->rows++;
-        b->meta[row] = Row{done, user, side};
+        b->meta[row] = Row{done, user, side, false};
         if (side != 19) b->all19 = false;
         b->writers++;
         outstanding_++;
@@ -173,6 +193,9 @@ public:
         if (row == 0) cv_.notify_one();
         std::memcpy(b->h_planes + (size_t)row * kPos, planes,
                     (size_t)32 * side * side);
+        if (side == 19)
+            b->meta[row].packed =
+                pack_row(planes, b->h_bits + (size_t)row * kP);
         // The dispatcher takes a closed batch once its last copy is in:
         // the producer that finishes it wakes the dispatcher (a batch still
         // open is taken when it fills or on the dispatcher's timer).
@@ -251,6 +274,10 @@ private:
         if (e == hipSuccess)
             e = hipHostMalloc(reinterpret_cast<void **>(&b.h_out),
                               sizeof(float) * cap_ * kOut,
+                              hipHostMallocCoherent);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void **>(&b.h_bits),
+                              sizeof(uint32_t) * cap_ * kP,
                               hipHostMallocCoherent);
         if (e == hipSuccess)
             e = hipHostMalloc(reinterpret_cast<void **>(&b.h_table),
@@ -342,6 +369,7 @@ private:
         if (b.h_raw) hipHostFree(b.h_raw);
         if (b.h_out) hipHostFree(b.h_out);
         if (b.h_table) hipHostFree(b.h_table);
+        if (b.h_bits) hipHostFree(b.h_bits);
         if (b.d_planes) hipFree(b.d_planes);
         if (b.d_raw) hipFree(b.d_raw);
         if (b.d_out) hipFree(b.d_out);
@@ -352,6 +380,7 @@ private:
         b.h_planes = b.d_planes = b.h_table = b.d_table = nullptr;
         b.h_raw = b.h_out = b.d_raw = b.d_out = nullptr;
         b.d_in_off = b.d_out_off = nullptr;
+        b.h_bits = nullptr;
         b.stream = nullptr;
         b.done = nullptr;
         b.meta.clear();
@@ -508,10 +537,14 @@ private:
             // on either side of a small batch a copy's latency is that of
             // several of its convs. (The layered kernels take the batch
             // size as an argument, not from the batch table.)
+            // The stem reads the packed rows when every row has them.
+            bool packed = true;
+            for (int i = 0; i < n; i++) packed = packed && b.meta[i].packed;
             std::string msg;
             if (run_net(net, b.net, precision_, n, b.h_planes, b.h_raw,
-                        b.h_out, ctx_->temperature, b.ws, b.stream,
-                        &msg) != LZ_OK)
+                        b.h_out, ctx_->temperature, b.ws, b.stream, &msg,
+                        nullptr, nullptr, nullptr,
+                        packed ? b.h_bits : nullptr) != LZ_OK)
                 e = hipErrorLaunchFailure;
         } else {
             e = hipMemcpyAsync(b.d_table, b.h_table, tbytes,

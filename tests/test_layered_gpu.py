@@ -107,3 +107,38 @@ def test_layered_latency(be):
         f"n={n} fused {res[(n, 0)]:.0f} layered {res[(n, 64)]:.0f}"
         for n in (1, 16, 64)), flush=True)
     assert res[(1, 64)] < res[(1, 0)]
+
+
+def test_queue_rows_that_are_not_binary_go_in_as_bytes(be):
+    """The queue packs 19x19 rows one bit per plane for the layered stem; a
+    batch with a plane byte other than 0 / 1 goes in as bytes, so it still
+    equals a direct forward of the same bytes."""
+    import leela_amd
+    from leela_amd.backend import DONE_FN
+    p = leela_amd.PREC_FP16
+    planes = make_batch(6, seed0=991)
+    planes[2, 7, 100] = 2  # not a binary plane value
+    be.set_layered_max(64)
+    want_raw, want = be.forward(leela_amd.NET_POLICY, p, planes)
+    got = np.full((6, 361), np.nan, np.float32)
+    status = np.full(6, -99, np.int32)
+
+    def done(user, st, raw, out):
+        i = int(user or 0)
+        status[i] = st
+        if st == 0:
+            got[i] = np.ctypeslib.as_array(out, (361,))
+    cb = DONE_FN(done)
+    be.queue_start(p, 64, 2000)
+    try:
+        for i in range(6):
+            be.submit(leela_amd.NET_POLICY, planes[i], cb, i)
+        be.join()
+    finally:
+        be.queue_stop()
+    assert (status == 0).all()
+    np.testing.assert_array_equal(got, want)
+    # and the 2 matters: the same row with a 1 there differs
+    planes[2, 7, 100] = 1
+    _, other = be.forward(leela_amd.NET_POLICY, p, planes)
+    assert not np.array_equal(other[2], want[2])
