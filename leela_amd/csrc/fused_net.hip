@@ -928,23 +928,48 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
     // The 32 binary planes with a 2-wide zero halo (as fused_net stages
     // them).
     constexpr int rows_in = 2 + (S + 2);
-    for (int i = tid; i < rows_in * pin * 4; i += kWN * 64) {
-        const int pp = i >> 2, cg8 = (i & 3) * 8;
-        const int row = pp / pin - 2, xx = pp % pin - 2;
-        const bool in = row >= 0 && row < S && xx >= 0 && xx < S;
-        v8 v;
-        if (wb) {
-            const uint32_t word = in ? wb[row * S + xx] >> cg8 : 0u;
+    if (wb) {
+        // Packed rows (host memory): each thread's words are all loaded
+        // before any is used, one round trip instead of one per pixel.
+        constexpr int kPix = rows_in * pin, kIter = (kPix + 255) / 256;
+        static_assert(kWN * 64 == 256, "stem staging");
+        uint32_t word[kIter];
 #pragma unroll
-            for (int e = 0; e < 8; e++) v[e] = (T)(float)((word >> e) & 1u);
-        } else {
+        for (int it = 0; it < kIter; it++) {
+            const int pp = tid + it * 256;
+            const int row = pp / pin - 2, xx = pp % pin - 2;
+            const bool in =
+                pp < kPix && row >= 0 && row < S && xx >= 0 && xx < S;
+            word[it] = in ? wb[row * S + xx] : 0u;
+        }
+#pragma unroll
+        for (int it = 0; it < kIter; it++) {
+            const int pp = tid + it * 256;
+            if (pp >= kPix) continue;
+            const int row = pp / pin - 2, xx = pp % pin - 2;
+#pragma unroll
+            for (int cg8 = 0; cg8 < 32; cg8 += 8) {
+                v8 v;
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    v[e] = (T)(float)((word[it] >> (cg8 + e)) & 1u);
+                *reinterpret_cast<v8 *>(s_act + (row + 2) * rp_in +
+                                        (xx + 2) * kInPS + cg8) = v;
+            }
+        }
+    } else {
+        for (int i = tid; i < rows_in * pin * 4; i += kWN * 64) {
+            const int pp = i >> 2, cg8 = (i & 3) * 8;
+            const int row = pp / pin - 2, xx = pp % pin - 2;
+            const bool in = row >= 0 && row < S && xx >= 0 && xx < S;
+            v8 v;
 #pragma unroll
             for (int e = 0; e < 8; e++)
                 v[e] = (T)(in ? (float)s_pl[(cg8 + e) * kP + row * S + xx]
                               : 0.0f);
+            *reinterpret_cast<v8 *>(s_act + (row + 2) * rp_in +
+                                    (xx + 2) * kInPS + cg8) = v;
         }
-        *reinterpret_cast<v8 *>(s_act + (row + 2) * rp_in + (xx + 2) * kInPS +
-                                cg8) = v;
     }
     __syncthreads();
     f32x4 acc[kLMT][kNT];
