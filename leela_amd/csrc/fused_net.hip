@@ -1152,7 +1152,6 @@ hipError_t layered_run(int which, const uint8_t *planes, const FusedArgs &a,
         buf[1] = val + (size_t)kLayeredMaxBoards * ltile<80>();
     }
     const int *cnt = a.counts;
-    if (!a.h0) return hipErrorInvalidValue;  // the head planes
     if (which == 0) {
         if (a.nconv != 13 || a.c != 128 || a.m1 != 96)
             return hipErrorInvalidValue;
@@ -1169,7 +1168,8 @@ hipError_t layered_run(int which, const uint8_t *planes, const FusedArgs &a,
                            temperature);
         return hipGetLastError();
     }
-    if (a.nconv != 12 || a.c != 64 || a.m1 != 64) return hipErrorInvalidValue;
+    if (a.nconv != 12 || a.c != 64 || a.m1 != 64 || !a.h0)
+        return hipErrorInvalidValue;  // (h0: the value head planes)
     hipLaunchKernelGGL((layered_stem<T, 80, 64>), dim3(n * 2), blk, 0, s,
                        planes, bits, a.w[0], a.b[0], buf[0], n, cnt);
     for (int l = 1; l <= 10; l++)
