@@ -230,58 +230,76 @@ __device__ __forceinline__ void conv_tile(const T *act,
 // unrolled, the A fragments prefetched PF k-steps ahead so that more of
 // their L2 latency is covered, the B fragments one step ahead. Every
 // accumulator sees the same MFMAs in the same order as in conv_tile.
-template <typename T, int F, int CIN, int MT, int MTW, int PS, int NT, int PF>
-__device__ __forceinline__ void conv_tile_deep(
-    const T *act, const typename Op<T>::v8 *__restrict__ wpk,
-    f32x4 (&acc)[MTW][NT], const int (&pb)[NT], int rp, int mt0, int lane,
-    const float *__restrict__ bias = nullptr) {
+// prefetch() issues the bias loads and the first PF steps' A fragments,
+// so that a kernel can overlap them with the staging of its input tile;
+// run() then does the convolution.
+template <typename T, int F, int CIN, int MT, int MTW, int PF>
+struct DeepConv {
     typedef typename Op<T>::v8 v8;
-    constexpr int CB = CIN / 32;
-    constexpr int STEPS = F * F * CB;
-    const int g8 = (lane >> 4) * 8;
-#pragma unroll
-    for (int m = 0; m < MTW; m++) {
-        f32x4 init = {0.f, 0.f, 0.f, 0.f};
-        if (bias) {
-            const int o0 = (mt0 + m) * 16 + (lane >> 4) * 4;
-#pragma unroll
-            for (int r = 0; r < 4; r++) init[r] = bias[o0 + r];
-        }
-#pragma unroll
-        for (int t = 0; t < NT; t++) acc[m][t] = init;
-    }
-    const v8 *wl = wpk + mt0 * 64 + lane;
-    auto b_off = [&](int s) {
-        const int tap = s / CB, cb = s % CB;
-        return (tap / F) * rp + (tap % F) * PS + cb * 32 + g8;
-    };
-    v8 a[PF][MTW], b[2][NT];
-#pragma unroll
-    for (int p = 0; p < PF; p++)
-        if (p < STEPS) {
-#pragma unroll
-            for (int m = 0; m < MTW; m++) a[p][m] = wl[(p * MT + m) * 64];
-        }
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-        b[0][t] = *reinterpret_cast<const v8 *>(act + pb[t] + b_off(0));
-#pragma unroll
-    for (int s = 0; s < STEPS; s++) {
-        if (s + 1 < STEPS) {
-#pragma unroll
-            for (int t = 0; t < NT; t++)
-                b[(s + 1) & 1][t] =
-                    *reinterpret_cast<const v8 *>(act + pb[t] + b_off(s + 1));
-        }
+    static constexpr int CB = CIN / 32;
+    static constexpr int STEPS = F * F * CB;
+    const v8 *wl;
+    v8 a[PF][MTW];
+    f32x4 init[MTW];
+
+    __device__ __forceinline__ void prefetch(const v8 *__restrict__ wpk,
+                                             int mt0, int lane,
+                                             const float *__restrict__ bias) {
 #pragma unroll
         for (int m = 0; m < MTW; m++) {
+            init[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (bias) {
+                const int o0 = (mt0 + m) * 16 + (lane >> 4) * 4;
 #pragma unroll
-            for (int t = 0; t < NT; t++)
-                acc[m][t] = Op<T>::mfma(a[s % PF][m], b[s & 1][t], acc[m][t]);
-            if (s + PF < STEPS) a[s % PF][m] = wl[((s + PF) * MT + m) * 64];
+                for (int r = 0; r < 4; r++) init[m][r] = bias[o0 + r];
+            }
+        }
+        wl = wpk + mt0 * 64 + lane;
+#pragma unroll
+        for (int p = 0; p < PF; p++)
+            if (p < STEPS) {
+#pragma unroll
+                for (int m = 0; m < MTW; m++) a[p][m] = wl[(p * MT + m) * 64];
+            }
+    }
+
+    template <int PS, int NT>
+    __device__ __forceinline__ void run(const T *act, f32x4 (&acc)[MTW][NT],
+                                        const int (&pb)[NT], int rp,
+                                        int lane) {
+        const int g8 = (lane >> 4) * 8;
+#pragma unroll
+        for (int m = 0; m < MTW; m++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[m][t] = init[m];
+        auto b_off = [&](int s) {
+            const int tap = s / CB, cb = s % CB;
+            return (tap / F) * rp + (tap % F) * PS + cb * 32 + g8;
+        };
+        v8 b[2][NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+            b[0][t] = *reinterpret_cast<const v8 *>(act + pb[t] + b_off(0));
+#pragma unroll
+        for (int s = 0; s < STEPS; s++) {
+            if (s + 1 < STEPS) {
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+                    b[(s + 1) & 1][t] = *reinterpret_cast<const v8 *>(
+                        act + pb[t] + b_off(s + 1));
+            }
+#pragma unroll
+            for (int m = 0; m < MTW; m++) {
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+                    acc[m][t] =
+                        Op<T>::mfma(a[s % PF][m], b[s & 1][t], acc[m][t]);
+                if (s + PF < STEPS)
+                    a[s % PF][m] = wl[((s + PF) * MT + m) * 64];
+            }
         }
     }
-}
+};
 
 // ELU of acc (whose bias the conv folded into the accumulator start),
 // written into the trunk tile at the lane's pixel offsets po[t] (-1:
@@ -913,6 +931,7 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
     // tail through registers (no read past the board).
     const uint8_t *src = planes + (size_t)board * kLPlaneBytes;
     const uint32_t *wb = bits ? bits + (size_t)board * kP : nullptr;
+    DeepConv<T, 5, 32, MT1, kLMT, kLayeredPF> dc;
     constexpr int kFull = kLPlaneBytes / 1024, kTail = kLPlaneBytes % 1024;
     static_assert(kTail % 16 == 0 && kLPlaneBytes % 16 == 0, "plane copy");
     if (!wb) {
@@ -920,6 +939,7 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
         if (tid < kTail / 16)
             reinterpret_cast<uint4 *>(s_pl + kFull * 1024)[tid] =
                 reinterpret_cast<const uint4 *>(src + kFull * 1024)[tid];
+        dc.prefetch(static_cast<const v8 *>(w), cg * kLMT, lane, bias);
     }
     int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
     lgeom<PS>(wave * kNT, lane, pb_in, pb, po, pix);
@@ -942,6 +962,8 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
                 pp < kPix && row >= 0 && row < S && xx >= 0 && xx < S;
             word[it] = in ? wb[row * S + xx] : 0u;
         }
+        // (after the words: waiting for them does not wait for these)
+        dc.prefetch(static_cast<const v8 *>(w), cg * kLMT, lane, bias);
 #pragma unroll
         for (int it = 0; it < kIter; it++) {
             const int pp = tid + it * 256;
@@ -973,9 +995,7 @@ __global__ __launch_bounds__(kWN * 64) void layered_stem(
     }
     __syncthreads();
     f32x4 acc[kLMT][kNT];
-    conv_tile_deep<T, 5, 32, MT1, kLMT, kInPS, kNT, kLayeredPF>(
-        s_act, static_cast<const v8 *>(w), acc, pb_in, rp_in, cg * kLMT, lane,
-        bias);
+    dc.template run<kInPS, kNT>(s_act, acc, pb_in, rp_in, lane);
     store_tile<T, kLMT>(out_tiles + (size_t)board * ltile<PS>(), acc,
                         cg * kLMT, po, lane);
 }
@@ -1050,14 +1070,14 @@ __global__ __launch_bounds__(kWN * 64) void layered_conv(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     band_stage<T, PS>(in_tiles + (size_t)board * TILE, b, s_act, tid, wave,
                       lane);
+    DeepConv<T, 3, CIN, MT, MTW, kLayeredPF> dc;
+    dc.prefetch(static_cast<const v8 *>(w), wave * MTW, lane, bias);
     int pb[kBandNT], po[kBandNT], pix[kBandNT];
     band_geom<PS>(b, lane, pb, po, pix);
     wait_lds_copies();
     __syncthreads();
     f32x4 acc[MTW][kBandNT];
-    conv_tile_deep<T, 3, CIN, MT, MTW, PS, kBandNT, kLayeredPF>(
-        s_act, static_cast<const v8 *>(w), acc, pb, lrp<PS>(), wave * MTW,
-        lane, bias);
+    dc.template run<PS, kBandNT>(s_act, acc, pb, lrp<PS>(), lane);
     store_tile<T, MTW, kBandNT>(out_tiles + (size_t)board * TILE, acc,
                                 wave * MTW, po, lane);
 }
@@ -1077,14 +1097,16 @@ __global__ __launch_bounds__(kWN * 64) void layered_head(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     band_stage<T, PS>(in_tiles + (size_t)board * TILE, b, s_act, tid, wave,
                       lane);
+    DeepConv<T, 3, C, 1, 1, kLayeredPF> dc;
+    if (wave == 0)
+        dc.prefetch(static_cast<const v8 *>(w), 0, lane, nullptr);
     int pb[kBandNT], po[kBandNT], pix[kBandNT];
     band_geom<PS>(b, lane, pb, po, pix);
     wait_lds_copies();
     __syncthreads();
     if (wave != 0) return;
     f32x4 acc[1][kBandNT];
-    conv_tile_deep<T, 3, C, 1, 1, PS, kBandNT, kLayeredPF>(
-        s_act, static_cast<const v8 *>(w), acc, pb, lrp<PS>(), 0, lane);
+    dc.template run<PS, kBandNT>(s_act, acc, pb, lrp<PS>(), lane);
     const float hb = hbias[0];
     if (lane < 16) {
 #pragma unroll
@@ -1113,13 +1135,14 @@ __global__ __launch_bounds__(kWN * 64) void layered_policy_out(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     copy_to_lds(in_tiles + (size_t)board * TILE, s_act,
                 TILE * (int)sizeof(T) / 1024, wave, lane);
+    DeepConv<T, 3, C, 1, 1, kLayeredPF> dc;
+    dc.prefetch(static_cast<const v8 *>(w), 0, lane, nullptr);
     int pb_in[kNT], pb[kNT], po[kNT], pix[kNT];
     lgeom<PS>(wave * kNT, lane, pb_in, pb, po, pix);
     wait_lds_copies();
     __syncthreads();
     f32x4 acc[1][kNT];
-    conv_tile_deep<T, 3, C, 1, 1, PS, kNT, kLayeredPF>(
-        s_act, static_cast<const v8 *>(w), acc, pb, lrp<PS>(), 0, lane);
+    dc.template run<PS, kNT>(s_act, acc, pb, lrp<PS>(), lane);
     const float hb = hbias[0];
     if (lane < 16) {
 #pragma unroll
