@@ -22,10 +22,15 @@
 //    ragged graph otherwise — and copies the results back, all on the
 //    batch's own stream, then records an event and goes on to the next
 //    batch while the device works.
-//  * A completion thread waits on the batches' events in launch order,
-//    runs the callbacks (never inside the HIP runtime) and returns the
-//    buffer to the pool. Up to kSets batches are being filled, launched or
-//    evaluated at once.
+//  * A batch of at most layered_max dense 19x19 rows in a 16-bit form
+//    skips the graph and the copies: the layered kernels, launched
+//    directly, read its rows from the staging buffer (coherent host
+//    memory; the stems read the rows packed one bit per plane, which the
+//    producers write beside the bytes) and write the results there.
+//  * A completion thread waits on the batches' events in launch order
+//    (polling, then blocking), runs the callbacks (never inside the HIP
+//    runtime) and returns the buffer to the pool. Up to kSets batches are
+//    being filled, launched or evaluated at once.
 //  * The 16-bit and FP8 kernels read the row count from device memory, so a
 //    graph replayed on a part-filled bucket evaluates exactly its rows; the
 //    FP32 graphs evaluate the whole bucket (each row's result is the same
