@@ -198,7 +198,11 @@ public:
         if (row == 0) cv_.notify_one();
         std::memcpy(b->h_planes + (size_t)row * kPos, planes,
                     (size_t)32 * side * side);
-        if (side == 19)
+        // Packed only where the batch may run the layered kernels (16-bit
+        // forms, within layered_max rows); a batch with an unpacked row
+        // goes in as bytes.
+        if (side == 19 && row < ctx_->net[net].layered_max &&
+            (precision_ == LZ_PREC_BF16 || precision_ == LZ_PREC_FP16))
             b->meta[row].packed =
                 pack_row(planes, b->h_bits + (size_t)row * kP);
         // The dispatcher takes a closed batch once its last copy is in:
