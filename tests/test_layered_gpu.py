@@ -109,19 +109,22 @@ def test_layered_latency(be):
     assert res[(1, 64)] < res[(1, 0)]
 
 
-def test_queue_rows_that_are_not_binary_go_in_as_bytes(be):
-    """The queue packs 19x19 rows one bit per plane for the layered stem; a
-    batch with a plane byte other than 0 / 1 goes in as bytes, so it still
-    equals a direct forward of the same bytes."""
+@pytest.mark.parametrize("n,max_batch", [(6, 64), (100, 256)])
+def test_queue_rows_that_are_not_binary_go_in_as_bytes(be, n, max_batch):
+    """The queue packs the rows of small batches one bit per plane for the
+    layered stem; a batch with a plane byte other than 0 / 1 goes in as
+    bytes, so it still equals a direct forward of the same bytes, as does a
+    batch too large for the layered kernels (a graph replay of bytes)."""
     import leela_amd
     from leela_amd.backend import DONE_FN
     p = leela_amd.PREC_FP16
-    planes = make_batch(6, seed0=991)
+    print(f"case: n={n} max_batch={max_batch}", flush=True)
+    planes = make_batch(n, seed0=991)
     planes[2, 7, 100] = 2  # not a binary plane value
     be.set_layered_max(64)
     want_raw, want = be.forward(leela_amd.NET_POLICY, p, planes)
-    got = np.full((6, 361), np.nan, np.float32)
-    status = np.full(6, -99, np.int32)
+    got = np.full((n, 361), np.nan, np.float32)
+    status = np.full(n, -99, np.int32)
 
     def done(user, st, raw, out):
         i = int(user or 0)
@@ -129,9 +132,9 @@ def test_queue_rows_that_are_not_binary_go_in_as_bytes(be):
         if st == 0:
             got[i] = np.ctypeslib.as_array(out, (361,))
     cb = DONE_FN(done)
-    be.queue_start(p, 64, 2000)
+    be.queue_start(p, max_batch, 2000)
     try:
-        for i in range(6):
+        for i in range(n):
             be.submit(leela_amd.NET_POLICY, planes[i], cb, i)
         be.join()
     finally:
