@@ -538,6 +538,9 @@ def main() -> int:
     nb_threads = 16
     be.queue_start(p, max_batch=nb_threads, max_wait_us=200)
     try:
+        # One untimed run first: the queue's threads, buffers and the
+        # layered kernels' first launches warm up.
+        Network(be, p).benchmark(positions.Position.random(1), nb_threads)
         nb_policy, nb_value = Network(be, p).benchmark(
             positions.Position.random(1), nb_threads)
     finally:
